@@ -1,0 +1,116 @@
+// epet_internal.hpp -- shared host/device definitions of libepet.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/epet.h"
+
+namespace epet {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const char* file, int line, const std::string& msg);
+
+#define EPET_FAIL(code, msg) return ::epet::fail((code), __FILE__, __LINE__, (msg))
+#define EPET_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return ::epet::fail(EPET_ERR_HIP, __FILE__, __LINE__,                   \
+                          std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+#define EPET_CHECK(cond, code, msg) \
+  do { if (!(cond)) EPET_FAIL((code), (msg)); } while (0)
+#define EPET_TRY(call) \
+  do { int _r = (call); if (_r != EPET_OK) return _r; } while (0)
+
+// ---------------------------------------------------------------- geometry
+// Everything a generation kernel needs, passed by value in the kernel-argument
+// segment (read-only, scalar-cache resident: the AMD home of per-launch
+// uniform data that CUDA keeps in __constant__ banks).
+struct Geo {
+  // measurement setup (MeasurementSetup.hpp:129-139 fields)
+  float pos0x, pos1x;
+  int na;
+  float da;
+  int n0z, n0y, n1z, n1y;
+  float segx, segy, segz;
+  // voxel grid (VoxelGrid.hpp:88-91)
+  float go[3], gd[3];
+  int gn[3];
+  // derived
+  int V;                 // voxels
+  int nch;               // channels
+  int P1, P2, P3, P4;    // channel-id radix products n1y, n1y*n1z, .., all four
+  float sumRadii;        // getWorkqueue.hpp:54-56
+  float halfDiag2;       // (voxel half diagonal)^2 for the conservative ray test
+  int nrays;
+  uint32_t key0, key1;   // Philox key = seed
+  float sin_[EPET_MAX_ANGLES];
+  float cos_[EPET_MAX_ANGLES];
+};
+
+}  // namespace epet
+
+// ---------------------------------------------------------------- context
+struct epet_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool have_geo = false;
+  epet::Geo geo{};
+  int nrays = 8;
+  uint64_t seed = 0;
+  // small pinned host buffer for synchronous scalar reads
+  void* pinned = nullptr;
+  // reusable device scratch
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int num_cus = 256;
+};
+
+struct epet_matrix {
+  int m = 0, n = 0;
+  long long nnz = 0;
+  int value_type = EPET_VAL_F32;
+  int64_t* rowptr = nullptr;   // m+1
+  int* col = nullptr;          // nnz
+  void* val = nullptr;         // nnz x (4 | 2) bytes
+  float* rowscale = nullptr;   // m (reduced-precision storage only)
+  int device = 0;
+  bool owns = true;
+  // tuning chosen for this matrix (lanes per row of the row-parallel kernels)
+  int lanes_fwd = 0;
+};
+
+namespace epet {
+
+// device scratch owned by the context, grown on demand (stream-ordered use)
+int ctx_scratch(epet_ctx* ctx, size_t bytes, void** out);
+int ctx_geo(epet_ctx* ctx, Geo* g);   // copy of the geometry with nrays/seed
+int read_pinned(epet_ctx* ctx, void* dst, const void* src_dev, size_t bytes);
+
+// ---------------------------------------------------------------- scans
+// Exclusive prefix sums over n entries (device pointers), stream-ordered.
+// out may alias in.  Returns the total on the host when total != nullptr
+// (synchronises).
+int exclusive_scan_i32_to_i64(epet_ctx* ctx, const int* in, int64_t* out, long long n,
+                              long long* total);
+int exclusive_scan_i64(epet_ctx* ctx, const int64_t* in, int64_t* out, long long n,
+                       long long* total);
+int exclusive_scan_i32(epet_ctx* ctx, const int* in, int* out, long long n,
+                       long long* total);
+
+// ---------------------------------------------------------------- launch helpers
+inline unsigned grid_for(long long n, int block, long long max_blocks = 1 << 20) {
+  long long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (unsigned)b;
+}
+
+}  // namespace epet

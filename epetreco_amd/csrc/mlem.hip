@@ -1,0 +1,468 @@
+// mlem.hip -- mlemOperations drop-ins and the fused MLEM iteration.
+//
+// One MLEM iteration (tests/reco.cu:209-300) is four launches on the context's
+// stream, replayable as one hipGraph (no memset nodes; every clear is a
+// kernel):
+//   1. k_fwd_ratio : yhat_i = sum_k a_ik x_k, e_i = y_i / yhat_i (0 where
+//                    yhat_i = 0), FP64 log-likelihood sum y log yhat - yhat
+//   2. back projection c += A^T e (spmv.hip kernels)
+//   3. k_update    : x_j <- x_j c_j / (s_j + [s_j == 0]), FP64 sum of x
+//   4. k_normalise : x_j <- x_j / sum x, c <- 0, next sum slot <- 0, it++
+// Device state (iteration counter, sums, log-likelihoods, frame flags) lives in
+// one small buffer so that the same graph serves every iteration.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "epet_block.hpp"
+#include "epet_internal.hpp"
+
+namespace epet {
+
+int matrix_spmv(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
+                float beta, float* y);
+int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha, const float* X,
+                float beta, float* Y);
+int pick_lanes(long long nnz, int m);
+
+// ------------------------------------------------------------------ element-wise
+__global__ __launch_bounds__(256) void k_divides(float* __restrict__ res, const float* __restrict__ a,
+                                                 const float* __restrict__ b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) res[i] = a[i] / b[i];   // mlemOperations.hpp:22-32, unguarded
+}
+
+__global__ __launch_bounds__(256) void k_divides_multiplies(float* __restrict__ res,
+                                                            const float* __restrict__ a,
+                                                            const float* __restrict__ b,
+                                                            const float* __restrict__ c,
+                                                            long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    float dvs = c[i];
+    dvs += (float)(dvs == 0.0f);   // mlemOperations.hpp:48-50
+    res[i] = a[i] * b[i] / dvs;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scales(float* __restrict__ x, float alpha, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = alpha * x[i];
+}
+
+__device__ __forceinline__ double block_sum_f64(double v, double* s_w) {
+  v = wave_sum(v);
+  if (lane_id() == 0) s_w[wave_id()] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x / WAVE); w++) t += s_w[w];
+  return t;   // valid in thread 0
+}
+
+__global__ __launch_bounds__(256) void k_sum_f64(const float* __restrict__ x, long long n,
+                                                 double* __restrict__ out) {
+  __shared__ double s_w[256 / WAVE];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    acc += x[i];
+  const double t = block_sum_f64(acc, s_w);
+  if (threadIdx.x == 0) unsafeAtomicAdd(out, t);
+}
+
+__global__ void k_zero_f64(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ fused MLEM
+struct MlemState {
+  int it;          // iterations completed
+  int F;
+  double* sum;     // [2][F]
+  double* loglik;  // [niter][F]
+  int* active;     // [F]
+  int niter;
+};
+
+struct ValF32m { using T = float; static __device__ __forceinline__ float get(const T* p, long long k) { return __builtin_nontemporal_load(p + k); } };
+struct ValBF16m { using T = uint16_t; static __device__ __forceinline__ float get(const T* p, long long k) { return __uint_as_float((uint32_t)__builtin_nontemporal_load(p + k) << 16); } };
+struct ValF16m { using T = uint16_t; static __device__ __forceinline__ float get(const T* p, long long k) { return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k))); } };
+
+// Single frame: L lanes per row.
+template <typename VT, int L>
+__global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ col,
+                                                   const typename VT::T* __restrict__ val,
+                                                   const float* __restrict__ rowscale,
+                                                   const float* __restrict__ x,
+                                                   const float* __restrict__ y,
+                                                   float* __restrict__ e,
+                                                   const int* __restrict__ itp,
+                                                   double* __restrict__ loglik) {
+  __shared__ double s_w[256 / WAVE];
+  const int sub = threadIdx.x % L;
+  const long long r = (long long)blockIdx.x * (256 / L) + threadIdx.x / L;
+  double ll = 0.0;
+  if (r < m) {
+    const long long b = rowptr[r], end = rowptr[r + 1];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    long long k = b + sub;
+    for (; k + 3 * L < end; k += 4 * L) {
+      const int c0 = __builtin_nontemporal_load(col + k);
+      const int c1 = __builtin_nontemporal_load(col + k + L);
+      const int c2 = __builtin_nontemporal_load(col + k + 2 * L);
+      const int c3 = __builtin_nontemporal_load(col + k + 3 * L);
+      a0 += VT::get(val, k) * x[c0];
+      a1 += VT::get(val, k + L) * x[c1];
+      a2 += VT::get(val, k + 2 * L) * x[c2];
+      a3 += VT::get(val, k + 3 * L) * x[c3];
+    }
+    for (; k < end; k += L) a0 += VT::get(val, k) * x[__builtin_nontemporal_load(col + k)];
+    float acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    if (sub == 0) {
+      const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
+      const float yi = y[r];
+      e[r] = yh > 0.0f ? yi / yh : 0.0f;
+      if (yh > 0.0f) ll = (double)yi * log((double)yh) - (double)yh;
+    }
+  }
+  const double t = block_sum_f64(ll, s_w);
+  if (threadIdx.x == 0 && loglik) unsafeAtomicAdd(&loglik[*itp], t);
+}
+
+// Multi-frame: one wavefront per row, lane = (slot, frame).
+template <typename VT, int P>
+__global__ __launch_bounds__(256) void k_fwd_ratio_mf(int m, int F,
+                                                      const int64_t* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const typename VT::T* __restrict__ val,
+                                                      const float* __restrict__ rowscale,
+                                                      const float* __restrict__ X,
+                                                      const float* __restrict__ Y,
+                                                      float* __restrict__ E,
+                                                      const int* __restrict__ itp,
+                                                      double* __restrict__ loglik) {
+  constexpr int S = WAVE / P;
+  const int lane = lane_id();
+  const int f = lane % P, s = lane / P;
+  const long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id();
+  if (r >= m) return;
+  const long long b = rowptr[r], end = rowptr[r + 1];
+  float acc = 0.0f;
+  if (f < F)
+    for (long long k = b + s; k < end; k += S)
+      acc += VT::get(val, k) * X[(long long)__builtin_nontemporal_load(col + k) * F + f];
+#pragma unroll
+  for (int off = WAVE / 2; off >= P; off >>= 1) acc += __shfl_xor(acc, off, WAVE);
+  if (s == 0 && f < F) {
+    const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
+    const float yi = Y[r * F + f];
+    E[r * F + f] = yh > 0.0f ? yi / yh : 0.0f;
+    if (yh > 0.0f && loglik)
+      unsafeAtomicAdd(&loglik[(long long)(*itp) * F + f], (double)yi * log((double)yh) - (double)yh);
+  }
+}
+
+// x <- x c / (s + [s==0]) for active frames, FP64 sum per frame into slot it&1
+__global__ __launch_bounds__(256) void k_update(float* __restrict__ x, const float* __restrict__ c,
+                                                const float* __restrict__ s, long long n, int F,
+                                                const int* __restrict__ itp,
+                                                double* __restrict__ sum,
+                                                const int* __restrict__ active) {
+  __shared__ double s_w[256 / WAVE];
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = (*itp) & 1;
+  double v = 0.0;
+  int f = 0;
+  if (F == 1) {
+    if (idx < n) {
+      float dvs = s[idx];
+      dvs += (float)(dvs == 0.0f);
+      float xn = x[idx];
+      if (!active || active[0]) { xn = x[idx] * c[idx] / dvs; x[idx] = xn; }
+      v = xn;
+    }
+    const double t = block_sum_f64(v, s_w);
+    if (threadIdx.x == 0) unsafeAtomicAdd(&sum[slot], t);
+  } else {
+    if (idx < n * F) {
+      const long long j = idx / F;
+      f = (int)(idx % F);
+      float dvs = s[j];
+      dvs += (float)(dvs == 0.0f);
+      float xn = x[idx];
+      if (!active || active[f]) { xn = x[idx] * c[idx] / dvs; x[idx] = xn; }
+      unsafeAtomicAdd(&sum[slot * F + f], (double)xn);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_normalise(float* __restrict__ x, float* __restrict__ c,
+                                                   long long n, int F, const int* __restrict__ itp,
+                                                   double* __restrict__ sum,
+                                                   const int* __restrict__ active) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = (*itp) & 1;
+  if (idx < n * (long long)F) {
+    const int f = (F == 1) ? 0 : (int)(idx % F);
+    if (!active || active[f]) {
+      const double S = sum[slot * F + f];
+      x[idx] = x[idx] * (float)(1.0 / S);   // reco.cu:280-281
+    }
+    c[idx] = 0.0f;
+  }
+  if (idx < F) sum[(slot ^ 1) * F + idx] = 0.0;
+}
+
+__global__ void k_advance(int* itp) {
+  if (threadIdx.x == 0) *itp += 1;
+}
+
+__global__ __launch_bounds__(256) void k_fill_f32(float* __restrict__ p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_scale_by_sum(float* __restrict__ dst,
+                                                      const float* __restrict__ src, long long n,
+                                                      const double* __restrict__ sum) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] * (float)(1.0 / (*sum));   // reco.cu:182-183
+}
+
+template <typename VT>
+static int launch_fwd_ratio(hipStream_t st, const epet_matrix* A, int F, const float* x,
+                            const float* y, float* e, const int* itp, double* loglik) {
+  using T = typename VT::T;
+  const T* v = (const T*)A->val;
+  const int m = A->m;
+  if (m == 0) return EPET_OK;
+  if (F == 1) {
+    const int L = A->lanes_fwd > 0 ? A->lanes_fwd : pick_lanes(A->nnz, m);
+#define EPET_FR(LL)                                                                         \
+  k_fwd_ratio<VT, LL><<<grid_for(m, 256 / LL, 1ll << 31), 256, 0, st>>>(                     \
+      m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik)
+    switch (L) {
+      case 64: EPET_FR(64); break;
+      case 32: EPET_FR(32); break;
+      case 16: EPET_FR(16); break;
+      case 8: EPET_FR(8); break;
+      default: EPET_FR(4); break;
+    }
+#undef EPET_FR
+  } else {
+    const unsigned g = grid_for(m, 4, 1ll << 31);
+#define EPET_FM(P)                                                                            \
+  k_fwd_ratio_mf<VT, P><<<g, 256, 0, st>>>(m, F, A->rowptr, A->col, v, A->rowscale, x, y, e, \
+                                           itp, loglik)
+    if (F <= 2) EPET_FM(2);
+    else if (F <= 4) EPET_FM(4);
+    else if (F <= 8) EPET_FM(8);
+    else if (F <= 16) EPET_FM(16);
+    else if (F <= 32) EPET_FM(32);
+    else EPET_FM(64);
+#undef EPET_FM
+  }
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+// One MLEM iteration on `st` (capturable).
+static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const float* y,
+                          const float* s, float* x, float* e, float* c, int* itp, double* sum,
+                          double* loglik, const int* active) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  switch (A->value_type) {
+    case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32m>(st, A, F, x, y, e, itp, loglik); break;
+    case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16m>(st, A, F, x, y, e, itp, loglik); break;
+    default: rc = launch_fwd_ratio<ValF16m>(st, A, F, x, y, e, itp, loglik); break;
+  }
+  if (rc != EPET_OK) return rc;
+  if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
+  else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
+  if (rc != EPET_OK) return rc;
+  const long long nv = (long long)A->n * F;
+  k_update<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(x, c, s, A->n, F, itp, sum, active);
+  k_normalise<<<grid_for(std::max<long long>(nv, F), 256, 1ll << 31), 256, 0, st>>>(x, c, A->n, F,
+                                                                                   itp, sum, active);
+  k_advance<<<1, 64, 0, st>>>(itp);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+}  // namespace epet
+
+using namespace epet;
+
+extern "C" {
+
+int epet_divides(epet_ctx* ctx, float* res, const float* a, const float* b, long long n) {
+  EPET_CHECK(ctx && (n == 0 || (res && a && b)), EPET_ERR_ARG, "null argument");
+  if (n > 0) k_divides<<<grid_for(n, 256, 1ll << 31), 256, 0, ctx->stream>>>(res, a, b, n);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+int epet_dividesMultiplies(epet_ctx* ctx, float* res, const float* a, const float* b,
+                           const float* c, long long n) {
+  EPET_CHECK(ctx && (n == 0 || (res && a && b && c)), EPET_ERR_ARG, "null argument");
+  if (n > 0)
+    k_divides_multiplies<<<grid_for(n, 256, 1ll << 31), 256, 0, ctx->stream>>>(res, a, b, c, n);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+int epet_scales(epet_ctx* ctx, float* x, float alpha, long long n) {
+  EPET_CHECK(ctx && (n == 0 || x), EPET_ERR_ARG, "null argument");
+  if (n > 0) k_scales<<<grid_for(n, 256, 1ll << 31), 256, 0, ctx->stream>>>(x, alpha, n);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+int epet_sum(epet_ctx* ctx, const float* x, long long n, double* out) {
+  EPET_CHECK(ctx && out && (n == 0 || x), EPET_ERR_ARG, "null argument");
+  double* d = nullptr;
+  EPET_HIP(hipMallocAsync((void**)&d, sizeof(double), ctx->stream));
+  k_zero_f64<<<1, 64, 0, ctx->stream>>>(d, 1);
+  if (n > 0) k_sum_f64<<<grid_for(n, 256, 2048), 256, 0, ctx->stream>>>(x, n, d);
+  EPET_HIP(hipGetLastError());
+  int rc = read_pinned(ctx, out, d, sizeof(double));
+  (void)hipFreeAsync(d, ctx->stream);
+  return rc;
+}
+
+int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const float* s_in,
+              float* x, int niter, int use_graph, double tol, double* loglik_out,
+              int* frame_iters) {
+  EPET_CHECK(ctx && A && y && s_in && x, EPET_ERR_ARG, "null argument");
+  EPET_CHECK(F >= 1 && F <= 64, EPET_ERR_UNSUPPORTED, "frames must be in [1, 64]");
+  EPET_CHECK(niter >= 0, EPET_ERR_ARG, "negative iteration count");
+  const long long nv = (long long)A->n * F;
+  hipStream_t own = nullptr;
+  hipStream_t saved = ctx->stream;
+  if (use_graph && ctx->stream == nullptr) {
+    // capture needs a non-null stream; order it after the caller's null-stream work
+    EPET_HIP(hipStreamSynchronize(nullptr));
+    EPET_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    ctx->stream = own;
+  }
+  hipStream_t st = ctx->stream;
+  // buffers: e (m F), c (n F), s (n), state
+  float *e = nullptr, *c = nullptr, *s = nullptr;
+  int* ibuf = nullptr;      // [it, active[F]]
+  double* dbuf = nullptr;   // [ssum, sum[2F], loglik[niter F]]
+  const size_t nd = 1 + 2 * (size_t)F + (size_t)std::max(1, niter) * F;
+  int rc = EPET_OK;
+  auto cleanup = [&]() {
+    if (e) (void)hipFreeAsync(e, st);
+    if (c) (void)hipFreeAsync(c, st);
+    if (s) (void)hipFreeAsync(s, st);
+    if (ibuf) (void)hipFreeAsync(ibuf, st);
+    if (dbuf) (void)hipFreeAsync(dbuf, st);
+    (void)hipStreamSynchronize(st);
+    if (own) { (void)hipStreamDestroy(own); ctx->stream = saved; }
+  };
+#define EPET_MLEM_HIP(call)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (call);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, std::string(#call) + ": " + hipGetErrorString(_e)); \
+      cleanup();                                                                         \
+      return rc;                                                                         \
+    }                                                                                    \
+  } while (0)
+  EPET_MLEM_HIP(hipMallocAsync((void**)&e, sizeof(float) * std::max<long long>(1, (long long)A->m * F), st));
+  EPET_MLEM_HIP(hipMallocAsync((void**)&c, sizeof(float) * std::max<long long>(1, nv), st));
+  EPET_MLEM_HIP(hipMallocAsync((void**)&s, sizeof(float) * std::max(1, A->n), st));
+  EPET_MLEM_HIP(hipMallocAsync((void**)&ibuf, sizeof(int) * (1 + F), st));
+  EPET_MLEM_HIP(hipMallocAsync((void**)&dbuf, sizeof(double) * nd, st));
+  int* itp = ibuf;
+  int* active = ibuf + 1;
+  double* ssum = dbuf;
+  double* sum = dbuf + 1;
+  double* ll = dbuf + 1 + 2 * F;
+  k_zero_f64<<<grid_for((long long)nd, 256), 256, 0, st>>>(dbuf, (int)nd);
+  k_fill_f32<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(c, nv, 0.0f);
+  std::vector<int> hi(1 + F, 1);
+  hi[0] = 0;
+  EPET_MLEM_HIP(hipMemcpyAsync(ibuf, hi.data(), sizeof(int) * (1 + F), hipMemcpyHostToDevice, st));
+  // s <- s / sum(s)  (reco.cu:182-183)
+  k_sum_f64<<<grid_for(A->n, 256, 2048), 256, 0, st>>>(s_in, A->n, ssum);
+  k_scale_by_sum<<<grid_for(A->n, 256, 1ll << 31), 256, 0, st>>>(s, s_in, A->n, ssum);
+  EPET_MLEM_HIP(hipGetLastError());
+  EPET_MLEM_HIP(hipStreamSynchronize(st));
+
+  const bool stopping = tol > 0.0 && frame_iters != nullptr;
+  std::vector<int> act(F, 1), iters(F, 0);
+  std::vector<double> hll((size_t)std::max(1, niter) * F, 0.0);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph && niter > 0) {
+    EPET_MLEM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = mlem_iteration(ctx, A, F, y, s, x, e, c, itp, sum, ll, active);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (rc != EPET_OK) { cleanup(); return rc; }
+    EPET_MLEM_HIP(ce);
+    EPET_MLEM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  for (int it = 0; it < niter; it++) {
+    if (exec) {
+      hipError_t ge = hipGraphLaunch(exec, st);
+      if (ge != hipSuccess) { rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "hipGraphLaunch"); break; }
+    } else {
+      rc = mlem_iteration(ctx, A, F, y, s, x, e, c, itp, sum, ll, active);
+      if (rc != EPET_OK) break;
+    }
+    if (stopping) {
+      // per-frame stop on the relative log-likelihood change
+      if (hipMemcpyAsync(hll.data() + (size_t)it * F, ll + (size_t)it * F, sizeof(double) * F,
+                         hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "log-likelihood readback");
+        break;
+      }
+      bool changed = false, any = false;
+      for (int f = 0; f < F; f++) {
+        if (!act[f]) continue;
+        iters[f] = it + 1;
+        if (it > 0) {
+          const double a = hll[(size_t)it * F + f], b = hll[(size_t)(it - 1) * F + f];
+          if (std::fabs(a - b) <= tol * std::fabs(b)) { act[f] = 0; changed = true; }
+        }
+        any = any || act[f];
+      }
+      if (changed) {
+        if (hipMemcpyAsync(active, act.data(), sizeof(int) * F, hipMemcpyHostToDevice, st) !=
+            hipSuccess) {
+          rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "frame flag upload");
+          break;
+        }
+      }
+      if (!any) break;
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc == EPET_OK && loglik_out && niter > 0) {
+    if (hipMemcpyAsync(loglik_out, ll, sizeof(double) * (size_t)niter * F, hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "log-likelihood readback");
+  }
+  if (frame_iters) {
+    for (int f = 0; f < F; f++) frame_iters[f] = stopping ? iters[f] : niter;
+  }
+  cleanup();
+#undef EPET_MLEM_HIP
+  return rc;
+}
+
+}  // extern "C"
