@@ -85,6 +85,14 @@ struct epet_matrix {
   bool owns = true;
   // tuning chosen for this matrix (lanes per row of the row-parallel kernels)
   int lanes_fwd = 0;
+  // back-projection plan (built on first use): voxel tiles of bp_tile ids,
+  // bp_T tiles, per-row entry offsets of every tile boundary (m x (bp_T+1)),
+  // partial images of bp_nblk row blocks
+  int bp_T = 0, bp_tile = 0, bp_nblk = 0;
+  uint32_t* bp_toff = nullptr;
+  float* bp_partial = nullptr;
+  bool bp_failed = false;
+  ~epet_matrix();
 };
 
 namespace epet {

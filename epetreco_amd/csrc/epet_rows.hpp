@@ -1,0 +1,101 @@
+// epet_rows.hpp -- value decoding and the wavefront row dot product shared by
+// the projection kernels.
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "epet_block.hpp"
+
+namespace epet {
+
+// stored value -> float (reduced-precision values are multiplied by the row
+// scale by the caller)
+struct ValF32 {
+  using T = float;
+  static __device__ __forceinline__ float get(const T* p, long long k) {
+    return __builtin_nontemporal_load(p + k);
+  }
+};
+struct ValBF16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float get(const T* p, long long k) {
+    return __uint_as_float((uint32_t)__builtin_nontemporal_load(p + k) << 16);
+  }
+};
+struct ValF16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float get(const T* p, long long k) {
+    return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
+  }
+};
+
+template <typename IDX>
+__device__ __forceinline__ int ldcol(const int* p, IDX k) {
+  return __builtin_nontemporal_load(p + k);
+}
+
+// ------------------------------------------------------------------ vector forward
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+template <typename VT> struct Vec4;
+template <> struct Vec4<ValF32> {
+  static __device__ __forceinline__ void get(const float* p, long long k, float v[4]) {
+    const f32x4 t = __builtin_nontemporal_load((const f32x4*)(p + k));
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  }
+};
+template <> struct Vec4<ValBF16> {
+  static __device__ __forceinline__ void get(const uint16_t* p, long long k, float v[4]) {
+    const u16x4 t = __builtin_nontemporal_load((const u16x4*)(p + k));
+    v[0] = __uint_as_float((uint32_t)t.x << 16); v[1] = __uint_as_float((uint32_t)t.y << 16);
+    v[2] = __uint_as_float((uint32_t)t.z << 16); v[3] = __uint_as_float((uint32_t)t.w << 16);
+  }
+};
+template <> struct Vec4<ValF16> {
+  static __device__ __forceinline__ void get(const uint16_t* p, long long k, float v[4]) {
+    const u16x4 t = __builtin_nontemporal_load((const u16x4*)(p + k));
+    v[0] = __half2float(__ushort_as_half(t.x)); v[1] = __half2float(__ushort_as_half(t.y));
+    v[2] = __half2float(__ushort_as_half(t.z)); v[3] = __half2float(__ushort_as_half(t.w));
+  }
+};
+
+// Per-lane partial of sum_k a_k x[col_k] over row entries [b, e) for a whole
+// wavefront: scalar head up to a 4-entry boundary, 16-byte (col) and 16/8-byte
+// (value) vector loads for the body, two quads in flight per lane, scalar tail.
+template <typename VT>
+__device__ __forceinline__ float row_dot_wave(long long b, long long e, int lane,
+                                              const int* __restrict__ col,
+                                              const typename VT::T* __restrict__ val,
+                                              const float* __restrict__ x) {
+  float acc0 = 0.0f, acc1 = 0.0f;
+  long long kh = (b + 3) & ~3ll;
+  if (kh > e) kh = e;
+  if (b + lane < kh) acc0 += VT::get(val, b + lane) * x[ldcol(col, b + lane)];
+  const long long nq = (e - kh) >> 2;
+  long long q = lane;
+  for (; q + WAVE < nq; q += 2 * WAVE) {
+    const long long k0 = kh + 4 * q, k1 = kh + 4 * (q + WAVE);
+    const i32x4 c0 = __builtin_nontemporal_load((const i32x4*)(col + k0));
+    const i32x4 c1 = __builtin_nontemporal_load((const i32x4*)(col + k1));
+    float v0[4], v1[4];
+    Vec4<VT>::get(val, k0, v0);
+    Vec4<VT>::get(val, k1, v1);
+    acc0 += v0[0] * x[c0.x] + v0[1] * x[c0.y] + v0[2] * x[c0.z] + v0[3] * x[c0.w];
+    acc1 += v1[0] * x[c1.x] + v1[1] * x[c1.y] + v1[2] * x[c1.z] + v1[3] * x[c1.w];
+  }
+  if (q < nq) {
+    const long long k0 = kh + 4 * q;
+    const i32x4 c0 = __builtin_nontemporal_load((const i32x4*)(col + k0));
+    float v0[4];
+    Vec4<VT>::get(val, k0, v0);
+    acc0 += v0[0] * x[c0.x] + v0[1] * x[c0.y] + v0[2] * x[c0.z] + v0[3] * x[c0.w];
+  }
+  const long long kt = kh + (nq << 2);
+  if (kt + lane < e) acc1 += VT::get(val, kt + lane) * x[ldcol(col, kt + lane)];
+  return acc0 + acc1;
+}
+
+}  // namespace epet

@@ -19,6 +19,7 @@
 
 #include "epet_block.hpp"
 #include "epet_internal.hpp"
+#include "epet_rows.hpp"
 
 namespace epet {
 
@@ -27,6 +28,7 @@ int matrix_spmv(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const 
 int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha, const float* X,
                 float beta, float* Y);
 int pick_lanes(long long nnz, int m);
+int bp_prepare(epet_ctx* ctx, epet_matrix* A);
 
 // ------------------------------------------------------------------ element-wise
 __global__ __launch_bounds__(256) void k_divides(float* __restrict__ res, const float* __restrict__ a,
@@ -89,11 +91,9 @@ struct MlemState {
   int niter;
 };
 
-struct ValF32m { using T = float; static __device__ __forceinline__ float get(const T* p, long long k) { return __builtin_nontemporal_load(p + k); } };
-struct ValBF16m { using T = uint16_t; static __device__ __forceinline__ float get(const T* p, long long k) { return __uint_as_float((uint32_t)__builtin_nontemporal_load(p + k) << 16); } };
-struct ValF16m { using T = uint16_t; static __device__ __forceinline__ float get(const T* p, long long k) { return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k))); } };
 
-// Single frame: L lanes per row.
+// Single frame, short rows: L lanes per row, grid-stride over rows; one FP64
+// log-likelihood atomic per workgroup.
 template <typename VT, int L>
 __global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restrict__ rowptr,
                                                    const int* __restrict__ col,
@@ -106,31 +106,54 @@ __global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restr
                                                    double* __restrict__ loglik) {
   __shared__ double s_w[256 / WAVE];
   const int sub = threadIdx.x % L;
-  const long long r = (long long)blockIdx.x * (256 / L) + threadIdx.x / L;
+  const long long stride = (long long)gridDim.x * (256 / L);
   double ll = 0.0;
-  if (r < m) {
+  for (long long r = (long long)blockIdx.x * (256 / L) + threadIdx.x / L; r < m; r += stride) {
     const long long b = rowptr[r], end = rowptr[r + 1];
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    float a0 = 0.0f, a1 = 0.0f;
     long long k = b + sub;
-    for (; k + 3 * L < end; k += 4 * L) {
-      const int c0 = __builtin_nontemporal_load(col + k);
-      const int c1 = __builtin_nontemporal_load(col + k + L);
-      const int c2 = __builtin_nontemporal_load(col + k + 2 * L);
-      const int c3 = __builtin_nontemporal_load(col + k + 3 * L);
-      a0 += VT::get(val, k) * x[c0];
-      a1 += VT::get(val, k + L) * x[c1];
-      a2 += VT::get(val, k + 2 * L) * x[c2];
-      a3 += VT::get(val, k + 3 * L) * x[c3];
+    for (; k + L < end; k += 2 * L) {
+      a0 += VT::get(val, k) * x[ldcol(col, k)];
+      a1 += VT::get(val, k + L) * x[ldcol(col, k + L)];
     }
-    for (; k < end; k += L) a0 += VT::get(val, k) * x[__builtin_nontemporal_load(col + k)];
-    float acc = (a0 + a1) + (a2 + a3);
+    if (k < end) a0 += VT::get(val, k) * x[ldcol(col, k)];
+    float acc = a0 + a1;
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
     if (sub == 0) {
       const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
       const float yi = y[r];
       e[r] = yh > 0.0f ? yi / yh : 0.0f;
-      if (yh > 0.0f) ll = (double)yi * log((double)yh) - (double)yh;
+      if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
+    }
+  }
+  const double t = block_sum_f64(ll, s_w);
+  if (threadIdx.x == 0 && loglik) unsafeAtomicAdd(&loglik[*itp], t);
+}
+
+// Single frame, long rows: one wavefront per row with vector loads.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_fwd_ratio_vec(int m, const int64_t* __restrict__ rowptr,
+                                                       const int* __restrict__ col,
+                                                       const typename VT::T* __restrict__ val,
+                                                       const float* __restrict__ rowscale,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ y,
+                                                       float* __restrict__ e,
+                                                       const int* __restrict__ itp,
+                                                       double* __restrict__ loglik) {
+  __shared__ double s_w[256 / WAVE];
+  const int lane = lane_id();
+  const long long nw = (long long)gridDim.x * (256 / WAVE);
+  double ll = 0.0;
+  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
+    const long long b = rowptr[r], end = rowptr[r + 1];
+    const float acc = wave_sum(row_dot_wave<VT>(b, end, lane, col, val, x));
+    if (lane == 0) {
+      const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
+      const float yi = y[r];
+      e[r] = yh > 0.0f ? yi / yh : 0.0f;
+      if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
     }
   }
   const double t = block_sum_f64(ll, s_w);
@@ -238,7 +261,7 @@ __global__ __launch_bounds__(256) void k_scale_by_sum(float* __restrict__ dst,
 }
 
 template <typename VT>
-static int launch_fwd_ratio(hipStream_t st, const epet_matrix* A, int F, const float* x,
+static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F, const float* x,
                             const float* y, float* e, const int* itp, double* loglik) {
   using T = typename VT::T;
   const T* v = (const T*)A->val;
@@ -246,9 +269,16 @@ static int launch_fwd_ratio(hipStream_t st, const epet_matrix* A, int F, const f
   if (m == 0) return EPET_OK;
   if (F == 1) {
     const int L = A->lanes_fwd > 0 ? A->lanes_fwd : pick_lanes(A->nnz, m);
+    const long long cap = 16ll * ncu;
+    if (L == 64 && A->lanes_fwd == 0) {
+      k_fwd_ratio_vec<VT><<<(unsigned)std::min<long long>((m + 3) / 4, cap), 256, 0, st>>>(
+          m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik);
+      EPET_HIP(hipGetLastError());
+      return EPET_OK;
+    }
 #define EPET_FR(LL)                                                                         \
-  k_fwd_ratio<VT, LL><<<grid_for(m, 256 / LL, 1ll << 31), 256, 0, st>>>(                     \
-      m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik)
+  k_fwd_ratio<VT, LL><<<(unsigned)std::min<long long>(((long long)m + 256 / LL - 1) / (256 / LL), cap), \
+                        256, 0, st>>>(m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik)
     switch (L) {
       case 64: EPET_FR(64); break;
       case 32: EPET_FR(32); break;
@@ -281,9 +311,9 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
   hipStream_t st = ctx->stream;
   int rc;
   switch (A->value_type) {
-    case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32m>(st, A, F, x, y, e, itp, loglik); break;
-    case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16m>(st, A, F, x, y, e, itp, loglik); break;
-    default: rc = launch_fwd_ratio<ValF16m>(st, A, F, x, y, e, itp, loglik); break;
+    case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+    case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+    default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
   }
   if (rc != EPET_OK) return rc;
   if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
@@ -400,6 +430,11 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   EPET_MLEM_HIP(hipGetLastError());
   EPET_MLEM_HIP(hipStreamSynchronize(st));
 
+  // back-projection plan (allocates) before any graph capture
+  if (F == 1) {
+    rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
+    if (rc != EPET_OK) { cleanup(); return rc; }
+  }
   const bool stopping = tol > 0.0 && frame_iters != nullptr;
   std::vector<int> act(F, 1), iters(F, 0);
   std::vector<double> hll((size_t)std::max(1, niter) * F, 0.0);

@@ -258,9 +258,40 @@ __global__ void k_set_last_i64(int64_t* p, const int64_t* v) {
   if (threadIdx.x == 0) *p = *v;
 }
 
+// Structure check of caller arrays: bit 1 = invalid (row pointers not
+// monotone / not spanning [0, nnz], column outside [0, n)); bit 2 = columns not
+// strictly ascending within a row.
+__global__ __launch_bounds__(256) void k_check_csr(int m, int n, long long nnz,
+                                                   const int64_t* __restrict__ rowptr,
+                                                   const int* __restrict__ col,
+                                                   int* __restrict__ flags) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const int64_t b = rowptr[r], e = rowptr[r + 1];
+  int f = 0;
+  if (r == 0 && b != 0) f |= 1;
+  if (r == m - 1 && e != nnz) f |= 1;
+  if (e < b || b < 0 || e > nnz) f |= 1;
+  else {
+    int prev = -1;
+    for (int64_t k = b; k < e; k++) {
+      const int c = col[k];
+      if (c < 0 || c >= n) { f |= 1; break; }
+      if (c <= prev) f |= 2;
+      prev = c;
+    }
+  }
+  if (f) atomicOr(flags, f);
+}
+
 }  // namespace epet
 
 using namespace epet;
+
+epet_matrix::~epet_matrix() {
+  if (bp_toff) (void)hipFree(bp_toff);
+  if (bp_partial) (void)hipFree(bp_partial);
+}
 
 namespace epet {
 int matrix_alloc(epet_matrix** out, int m, int n, long long nnz, int value_type) {
@@ -410,9 +441,24 @@ int epet_matrix_create(epet_ctx* ctx, int m, int n, long long nnz, const int64_t
                        const int* col, const float* val, epet_matrix** out) {
   EPET_CHECK(ctx && out && rowptr && (nnz == 0 || (col && val)), EPET_ERR_ARG, "null argument");
   EPET_CHECK(m >= 0 && n >= 0 && nnz >= 0, EPET_ERR_ARG, "bad sizes");
+  hipStream_t st = ctx->stream;
+  int hflags = 0;
+  if (m > 0) {
+    int* flags = nullptr;
+    EPET_HIP(hipMallocAsync((void**)&flags, sizeof(int), st));
+    k_fill_i32<<<1, 64, 0, st>>>(flags, 1, 0);
+    k_check_csr<<<grid_for(m, 256, 1ll << 31), 256, 0, st>>>(m, n, nnz, rowptr, col, flags);
+    EPET_HIP(hipGetLastError());
+    int r = read_pinned(ctx, &hflags, flags, sizeof(int));
+    (void)hipFreeAsync(flags, st);
+    EPET_TRY(r);
+    EPET_CHECK(!(hflags & 1), EPET_ERR_ARG,
+               "invalid CSR arrays (row pointers or column ids out of range)");
+  }
   epet_matrix* A = nullptr;
   EPET_TRY(matrix_alloc(&A, m, n, nnz, EPET_VAL_F32));
-  hipStream_t st = ctx->stream;
+  // the tiled back projection needs ascending columns within rows
+  if (hflags & 2) A->bp_failed = true;
   EPET_HIP(hipMemcpyAsync(A->rowptr, rowptr, sizeof(int64_t) * ((size_t)m + 1),
                           hipMemcpyDeviceToDevice, st));
   if (nnz) {

@@ -9,37 +9,14 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 
 #include "epet_block.hpp"
 #include "epet_internal.hpp"
+#include "epet_rows.hpp"
 
 namespace epet {
-
-// stored value -> float (reduced-precision values are multiplied by the row
-// scale by the caller)
-struct ValF32 {
-  using T = float;
-  static __device__ __forceinline__ float get(const T* p, long long k) {
-    return __builtin_nontemporal_load(p + k);
-  }
-};
-struct ValBF16 {
-  using T = uint16_t;
-  static __device__ __forceinline__ float get(const T* p, long long k) {
-    return __uint_as_float((uint32_t)__builtin_nontemporal_load(p + k) << 16);
-  }
-};
-struct ValF16 {
-  using T = uint16_t;
-  static __device__ __forceinline__ float get(const T* p, long long k) {
-    return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
-  }
-};
-
-template <typename IDX>
-__device__ __forceinline__ int ldcol(const int* p, IDX k) {
-  return __builtin_nontemporal_load(p + k);
-}
 
 // y[r] = alpha * scale_r * sum_k a_k x[col_k] + beta * y[r]
 template <typename VT, typename RP, int L>
@@ -154,6 +131,178 @@ __global__ __launch_bounds__(256) void k_spmm_t(int m, int F, const int64_t* __r
   }
 }
 
+// Forward projection, one wavefront per row, persistent grid-stride over rows.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const typename VT::T* __restrict__ val,
+                                                    const float* __restrict__ rowscale,
+                                                    const float* __restrict__ x, float alpha,
+                                                    float beta, float* __restrict__ y) {
+  const int lane = lane_id();
+  const long long nw = (long long)gridDim.x * (256 / WAVE);
+  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
+    const long long b = rowptr[r], e = rowptr[r + 1];
+    float acc = wave_sum(row_dot_wave<VT>(b, e, lane, col, val, x));
+    if (lane == 0) {
+      const float v = alpha * (rowscale ? rowscale[r] : 1.0f) * acc;
+      y[r] = (beta == 0.0f) ? v : v + beta * y[r];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ tiled back projection
+// The image is cut into tiles of BP_TILE consecutive voxel ids (64 KB of LDS).
+// Workgroup (row block, tile) accumulates a_ij e_i of its rows' entries inside
+// the tile with LDS float atomics and writes the tile into its row block's
+// partial image; a reduction sums the partial images.  Entry ranges of every
+// (row, tile) come from a per-matrix plan of tile-boundary offsets.
+constexpr int BP_TILE = 16384;
+constexpr int BP_BLOCK = 1024;
+
+__global__ __launch_bounds__(256) void k_bp_plan(int m, int T, int TS,
+                                                 const int64_t* __restrict__ rowptr,
+                                                 const int* __restrict__ col,
+                                                 uint32_t* __restrict__ toff) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)m * (T + 1)) return;
+  const long long r = idx / (T + 1);
+  const int t = (int)(idx % (T + 1));
+  const long long b = rowptr[r], e = rowptr[r + 1];
+  long long lo = b, hi = e;   // first entry with col >= t*TS
+  const long long target = (long long)t * TS;
+  if (t == T) lo = e;
+  else
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if ((long long)col[mid] < target) lo = mid + 1; else hi = mid;
+    }
+  toff[idx] = (uint32_t)(lo - b);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BP_BLOCK) void k_bp_tiled(int m, int n, int T, int rows_per_block,
+                                                       const int64_t* __restrict__ rowptr,
+                                                       const int* __restrict__ col,
+                                                       const typename VT::T* __restrict__ val,
+                                                       const float* __restrict__ rowscale,
+                                                       const uint32_t* __restrict__ toff,
+                                                       const float* __restrict__ e, float alpha,
+                                                       float* __restrict__ partial) {
+  __shared__ float acc[BP_TILE];
+  const int t = blockIdx.y;
+  const long long r0 = (long long)blockIdx.x * rows_per_block;
+  const long long r1 = min((long long)m, r0 + rows_per_block);
+  const int tile0 = t * BP_TILE;
+  const int tn = min(BP_TILE, n - tile0);
+  for (int j = threadIdx.x; j < BP_TILE; j += BP_BLOCK) acc[j] = 0.0f;
+  __syncthreads();
+  const int sub = threadIdx.x & 31;          // 32 lanes per row segment
+  const int seg = threadIdx.x >> 5;          // 32 segments per workgroup
+  for (long long r = r0 + seg; r < r1; r += BP_BLOCK / 32) {
+    float ev = e[r];
+    if (ev == 0.0f) continue;
+    ev *= alpha * (rowscale ? rowscale[r] : 1.0f);
+    const long long base = rowptr[r];
+    const long long lo = base + toff[r * (T + 1) + t];
+    const long long hi = base + toff[r * (T + 1) + t + 1];
+    long long k = lo + sub;
+    for (; k + 96 < hi; k += 128) {
+      const int c0 = ldcol(col, k), c1 = ldcol(col, k + 32);
+      const int c2 = ldcol(col, k + 64), c3 = ldcol(col, k + 96);
+      const float v0 = VT::get(val, k), v1 = VT::get(val, k + 32);
+      const float v2 = VT::get(val, k + 64), v3 = VT::get(val, k + 96);
+      // the plan puts every column of [lo, hi) inside the tile; the unsigned
+      // compare keeps LDS indices in range even for an inconsistent plan
+      const unsigned j0 = c0 - tile0, j1 = c1 - tile0, j2 = c2 - tile0, j3 = c3 - tile0;
+      if (j0 < BP_TILE) atomicAdd(&acc[j0], v0 * ev);
+      if (j1 < BP_TILE) atomicAdd(&acc[j1], v1 * ev);
+      if (j2 < BP_TILE) atomicAdd(&acc[j2], v2 * ev);
+      if (j3 < BP_TILE) atomicAdd(&acc[j3], v3 * ev);
+    }
+    for (; k < hi; k += 32) {
+      const unsigned j = ldcol(col, k) - tile0;
+      if (j < BP_TILE) atomicAdd(&acc[j], VT::get(val, k) * ev);
+    }
+  }
+  __syncthreads();
+  float* out = partial + (size_t)blockIdx.x * n + tile0;
+  for (int j = threadIdx.x; j < tn; j += BP_BLOCK) out[j] = acc[j];
+}
+
+// y[j] = beta * y[j] + sum over row blocks of partial[b][j]
+__global__ __launch_bounds__(256) void k_bp_reduce(int n, int nblk,
+                                                   const float* __restrict__ partial,
+                                                   float beta, float* __restrict__ y) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  int b = 0;
+  for (; b + 3 < nblk; b += 4) {
+    s0 += partial[(size_t)b * n + j];
+    s1 += partial[(size_t)(b + 1) * n + j];
+    s2 += partial[(size_t)(b + 2) * n + j];
+    s3 += partial[(size_t)(b + 3) * n + j];
+  }
+  for (; b < nblk; b++) s0 += partial[(size_t)b * n + j];
+  const float s = (s0 + s1) + (s2 + s3);
+  y[j] = (beta == 0.0f) ? s : beta * y[j] + s;
+}
+
+int bp_prepare(epet_ctx* ctx, epet_matrix* A) {
+  if (A->bp_toff || A->bp_failed) return EPET_OK;
+  if (A->m == 0 || A->n == 0) { A->bp_failed = true; return EPET_OK; }
+  const int T = (A->n + BP_TILE - 1) / BP_TILE;
+  const int target_wgs = 8 * ctx->num_cus;
+  int nblk = std::max(1, std::min(A->m, (target_wgs + T - 1) / T));
+  const size_t toff_bytes = sizeof(uint32_t) * (size_t)A->m * (T + 1);
+  const size_t part_bytes = sizeof(float) * (size_t)nblk * A->n;
+  if (hipMalloc((void**)&A->bp_toff, toff_bytes) != hipSuccess ||
+      hipMalloc((void**)&A->bp_partial, part_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (A->bp_toff) (void)hipFree(A->bp_toff);
+    A->bp_toff = nullptr;
+    A->bp_partial = nullptr;
+    A->bp_failed = true;   // fall back to the atomic kernel
+    return EPET_OK;
+  }
+  A->bp_T = T;
+  A->bp_tile = BP_TILE;
+  A->bp_nblk = nblk;
+  const long long nthreads = (long long)A->m * (T + 1);
+  k_bp_plan<<<grid_for(nthreads, 256, 1ll << 31), 256, 0, ctx->stream>>>(A->m, T, BP_TILE, A->rowptr,
+                                                                        A->col, A->bp_toff);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+template <typename VT>
+static int launch_bp_tiled(hipStream_t st, const epet_matrix* A, const float* x, float alpha,
+                           float beta, float* y) {
+  using T = typename VT::T;
+  const int rows_per_block = (A->m + A->bp_nblk - 1) / A->bp_nblk;
+  dim3 grid(A->bp_nblk, A->bp_T);
+  k_bp_tiled<VT><<<grid, BP_BLOCK, 0, st>>>(A->m, A->n, A->bp_T, rows_per_block, A->rowptr, A->col,
+                                            (const T*)A->val, A->rowscale, A->bp_toff, x, alpha,
+                                            A->bp_partial);
+  k_bp_reduce<<<grid_for(A->n, 256, 1ll << 31), 256, 0, st>>>(A->n, A->bp_nblk, A->bp_partial,
+                                                             beta, y);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+template <typename VT>
+static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha,
+                          float beta, float* y) {
+  using T = typename VT::T;
+  const long long want = ((long long)A->m + 3) / 4;
+  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, 16ll * ctx->num_cus));
+  k_spmv_n_vec<VT><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                              A->rowscale, x, alpha, beta, y);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
 // lanes per row from the average row length
 int pick_lanes(long long nnz, int m) {
   const double avg = m > 0 ? (double)nnz / m : 0.0;
@@ -205,8 +354,40 @@ static int launch_spmv(hipStream_t st, int op, int m, int n, long long nnz, cons
   return EPET_OK;
 }
 
+// Kernel choice: rows of >= 64 entries on average use the vectorised wavefront
+// forward; back projection uses the LDS-tiled kernel once its plan exists
+// (EPET_BP=atomic forces the atomic kernel, for comparison).
+static bool bp_atomic_forced() {
+  const char* e = getenv("EPET_BP");
+  return e && std::string(e) == "atomic";
+}
+
+template <typename VT>
+static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
+                         float beta, float* y) {
+  const double avg = A->m ? (double)A->nnz / A->m : 0.0;
+  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0)
+    return launch_fwd_vec<VT>(ctx, A, x, alpha, beta, y);
+  if (op == EPET_OP_T && !bp_atomic_forced()) {
+    EPET_TRY(bp_prepare(ctx, const_cast<epet_matrix*>(A)));
+    if (A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);
+  }
+  return launch_spmv<VT, int64_t>(ctx->stream, op, A->m, A->n, A->nnz, A->rowptr, A->col, A->val,
+                                  A->rowscale, x, alpha, beta, y, A->lanes_fwd);
+}
+
 int matrix_spmv(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
                 float beta, float* y) {
+  switch (A->value_type) {
+    case EPET_VAL_F32: return spmv_dispatch<ValF32>(ctx, A, op, alpha, x, beta, y);
+    case EPET_VAL_BF16: return spmv_dispatch<ValBF16>(ctx, A, op, alpha, x, beta, y);
+    case EPET_VAL_F16: return spmv_dispatch<ValF16>(ctx, A, op, alpha, x, beta, y);
+  }
+  EPET_FAIL(EPET_ERR_STATE, "unknown value type");
+}
+
+int matrix_spmv_basic(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
+                      float beta, float* y) {
   switch (A->value_type) {
     case EPET_VAL_F32:
       return launch_spmv<ValF32, int64_t>(ctx->stream, op, A->m, A->n, A->nnz, A->rowptr, A->col,

@@ -127,3 +127,53 @@ def test_spmm_against_fp64(gpu_ctx_factory, oracle, F):
         cr = oracle.csrmv(1, m, n, 1.0, val, rp, col, E[:, f].copy(), 0.0, np.zeros(n, np.float32))
         np.testing.assert_allclose(Y[:, f], yr, rtol=2e-5, atol=1e-6)
         np.testing.assert_allclose(C[:, f], cr, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,avg", [(40000, 900), (70001, 200), (16384, 50)])
+def test_spmv_multi_tile_back_projection(gpu_ctx_factory, oracle, n, avg):
+    """Back projection across several 16384-voxel LDS tiles, and the vector forward."""
+    from epetreco_amd.api import Matrix, OP_N, OP_T
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(n + avg)
+    m = 2500
+    rp, col, val = _random_csr(rng, m, n, avg)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    x = rng.random(n).astype(np.float32)
+    e = rng.random(m).astype(np.float32)
+    e[::7] = 0.0                       # rows skipped by the back projection
+    y0 = rng.random(n).astype(np.float32)
+    _say("multi-tile", n, avg, A.nnz)
+    y = A.spmv(OP_N, _t(x, torch.float32)).cpu().numpy()
+    yr = oracle.csrmv(0, m, n, 1.0, val, rp, col, x, 0.0, np.zeros(m, np.float32))
+    np.testing.assert_allclose(y, yr, rtol=2e-5, atol=1e-6)
+    for beta in (0.0, 1.0, 0.5):
+        c = A.spmv(OP_T, _t(e, torch.float32), _t(y0, torch.float32).clone(), alpha=1.5,
+                   beta=beta).cpu().numpy()
+        cr = oracle.csrmv(1, m, n, 1.5, val, rp, col, e, beta, y0)
+        np.testing.assert_allclose(c, cr, rtol=2e-5, atol=1e-6)
+
+
+def test_unsorted_columns_fall_back(gpu_ctx_factory, oracle):
+    """Columns not ascending within rows: still correct (atomic back projection)."""
+    from epetreco_amd.api import Matrix, OP_T
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(1)
+    m, n = 500, 30000
+    rp, col, val = _random_csr(rng, m, n, 100)
+    for i in range(m):
+        seg = col[rp[i]:rp[i + 1]]
+        rng.shuffle(seg)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    e = rng.random(m).astype(np.float32)
+    c = A.spmv(OP_T, _t(e, torch.float32)).cpu().numpy()
+    cr = oracle.csrmv(1, m, n, 1.0, val, rp, col, e, 0.0, np.zeros(n, np.float32))
+    np.testing.assert_allclose(c, cr, rtol=2e-5, atol=1e-6)
+
+
+def test_invalid_csr_rejected(gpu_ctx_factory):
+    from epetreco_amd.api import EpetError, Matrix
+    ctx = gpu_ctx_factory()
+    rp = torch.tensor([0, 2, 3], dtype=torch.int64)
+    col = torch.tensor([0, 5, 1], dtype=torch.int32)     # 5 >= n
+    with pytest.raises(EpetError):
+        Matrix.from_csr(ctx, rp, col, torch.ones(3), 4)
