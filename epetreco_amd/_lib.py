@@ -63,6 +63,7 @@ SIGNATURES = {
     "epet_matrix_select_rows": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.POINTER(P)]),
     "epet_matrix_convert": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.POINTER(P)]),
     "epet_matrix_colsum_f64": (ctypes.c_int, [P, P, P]),
+    "epet_matrix_transpose": (ctypes.c_int, [P, P]),
     "epet_cooSort": (ctypes.c_int, [P, P, P, P, ctypes.c_longlong]),
     "epet_convertCoo2Csr": (ctypes.c_int, [P, P, P, ctypes.c_longlong, ctypes.c_int]),
     "epet_convertCsr2Ecsr": (ctypes.c_int, [P, P, P, ctypes.c_int, P, ctypes.c_int]),

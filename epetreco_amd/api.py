@@ -326,6 +326,11 @@ class Matrix:
         check(lib().epet_matrix_convert(self.ctx.handle, self._h, value_type, ctypes.byref(h)))
         return Matrix(self.ctx, h)
 
+    def attach_transpose(self) -> "Matrix":
+        """Build the transposed copy used by the multi-frame back projection."""
+        check(lib().epet_matrix_transpose(self.ctx.handle, self._h))
+        return self
+
     def colsum_f64(self) -> torch.Tensor:
         s = torch.empty(self.n, dtype=torch.float64, device=self.ctx.device)
         check(lib().epet_matrix_colsum_f64(self.ctx.handle, self._h, _p(s)))

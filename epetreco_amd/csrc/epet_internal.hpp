@@ -92,6 +92,11 @@ struct epet_matrix {
   uint32_t* bp_toff = nullptr;
   float* bp_partial = nullptr;
   bool bp_failed = false;
+  bool cols_strict = true;   // columns strictly ascending within every row
+  // transposed copy (CSR of A^T: rows = voxels, entries (row index, value)),
+  // used by the multi-frame back projection; per-row scales of A are folded
+  // into the back-projected vector instead
+  epet_matrix* tr = nullptr;
   ~epet_matrix();
 };
 

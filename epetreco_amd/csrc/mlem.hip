@@ -193,38 +193,148 @@ __global__ __launch_bounds__(256) void k_fwd_ratio_mf(int m, int F,
   }
 }
 
+// ------------------------------------------------------------------ multi-frame v2
+// XCD-aware persistent mapping: workgroups are dispatched round-robin over the
+// 8 XCDs, so workgroup b works on the contiguous eighth (b mod 8) of the rows;
+// concurrently processed rows on one XCD are then neighbours in channel order
+// and share image rows in that XCD's L2.  (Only locality depends on this.)
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ void xcd_range(long long n, long long& beg, long long& end,
+                                          long long& first, long long& stride) {
+  const int xcd = blockIdx.x % NXCD;
+  const int kb = blockIdx.x / NXCD;
+  const int nb = (gridDim.x - xcd + NXCD - 1) / NXCD;   // workgroups on this XCD
+  const long long chunk = (n + NXCD - 1) / NXCD;
+  beg = min(n, xcd * chunk);
+  end = min(n, beg + chunk);
+  first = beg + (long long)kb * (256 / WAVE) + wave_id();
+  stride = (long long)nb * (256 / WAVE);
+}
+
+// Forward for F frames (frame-minor X), one wavefront per row: lanes load 64
+// (column, value) pairs at once, then lane (slot s, frame f) gathers row
+// X[col] of 64/P entries per step.  Epilogue: E = Y / Yhat (0 where Yhat = 0),
+// times the row scale when fold = 1 (the transposed back projection has no
+// row scales), and the FP64 log-likelihood per frame.
+template <typename VT, int P>
+__global__ __launch_bounds__(256) void k_fwd_mf2(int m, int F, const int64_t* __restrict__ rowptr,
+                                                 const int* __restrict__ col,
+                                                 const typename VT::T* __restrict__ val,
+                                                 const float* __restrict__ rowscale,
+                                                 const float* __restrict__ X,
+                                                 const float* __restrict__ Y,
+                                                 float* __restrict__ E, int fold,
+                                                 const int* __restrict__ itp,
+                                                 double* __restrict__ loglik) {
+  constexpr int S = WAVE / P;
+  __shared__ double s_ll[256 / WAVE][WAVE];
+  const int lane = lane_id();
+  const int f = lane % P, s = lane / P;
+  long long beg, end, r, stride;
+  xcd_range(m, beg, end, r, stride);
+  double ll = 0.0;
+  for (; r < end; r += stride) {
+    const long long b = rowptr[r], e = rowptr[r + 1];
+    float acc = 0.0f;
+    for (long long base = b; base < e; base += WAVE) {
+      const long long k = base + lane;
+      const int ci = k < e ? ldcol(col, k) : 0;
+      const float cv = k < e ? VT::get(val, k) : 0.0f;
+      const int nn = (int)min((long long)WAVE, e - base);
+#pragma unroll 4
+      for (int q = 0; q < nn; q += S) {
+        const int c = __shfl(ci, q + s, WAVE);
+        const float v = __shfl(cv, q + s, WAVE);
+        if (f < F) acc += v * X[(long long)c * F + f];
+      }
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off >= P; off >>= 1) acc += __shfl_xor(acc, off, WAVE);
+    if (s == 0 && f < F) {
+      const float sc = rowscale ? rowscale[r] : 1.0f;
+      const float yh = acc * sc;
+      const float yi = Y[r * F + f];
+      const float ev = yh > 0.0f ? yi / yh : 0.0f;
+      E[r * F + f] = fold ? ev * sc : ev;
+      if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
+    }
+  }
+  s_ll[wave_id()][lane] = (s == 0) ? ll : 0.0;
+  __syncthreads();
+  if (loglik && threadIdx.x < F) {
+    double t = 0.0;
+    for (int w = 0; w < 256 / WAVE; w++) t += s_ll[w][threadIdx.x];
+    unsafeAtomicAdd(&loglik[(long long)(*itp) * F + threadIdx.x], t);
+  }
+}
+
+// Back projection for F frames from the transposed matrix: one wavefront per
+// voxel j, C[j][f] = sum over the voxel's entries (i, v) of v * E[i][f].
+template <typename VT, int P>
+__global__ __launch_bounds__(256) void k_bp_gather_mf(int n, int F,
+                                                      const int64_t* __restrict__ tptr,
+                                                      const int* __restrict__ trow,
+                                                      const typename VT::T* __restrict__ tval,
+                                                      const float* __restrict__ E,
+                                                      float* __restrict__ C) {
+  constexpr int S = WAVE / P;
+  const int lane = lane_id();
+  const int f = lane % P, s = lane / P;
+  long long beg, end, j, stride;
+  xcd_range(n, beg, end, j, stride);
+  for (; j < end; j += stride) {
+    const long long b = tptr[j], e = tptr[j + 1];
+    float acc = 0.0f;
+    for (long long base = b; base < e; base += WAVE) {
+      const long long k = base + lane;
+      const int ri = k < e ? ldcol(trow, k) : 0;
+      const float rv = k < e ? VT::get(tval, k) : 0.0f;
+      const int nn = (int)min((long long)WAVE, e - base);
+#pragma unroll 4
+      for (int q = 0; q < nn; q += S) {
+        const int i = __shfl(ri, q + s, WAVE);
+        const float v = __shfl(rv, q + s, WAVE);
+        if (f < F) acc += v * E[(long long)i * F + f];
+      }
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off >= P; off >>= 1) acc += __shfl_xor(acc, off, WAVE);
+    if (s == 0 && f < F) C[j * F + f] = acc;
+  }
+}
+
 // x <- x c / (s + [s==0]) for active frames, FP64 sum per frame into slot it&1
+// (grid-stride; per-frame partial sums in LDS, one global atomic per
+// workgroup and frame)
 __global__ __launch_bounds__(256) void k_update(float* __restrict__ x, const float* __restrict__ c,
                                                 const float* __restrict__ s, long long n, int F,
                                                 const int* __restrict__ itp,
                                                 double* __restrict__ sum,
                                                 const int* __restrict__ active) {
-  __shared__ double s_w[256 / WAVE];
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double s_sum[64];
+  if (threadIdx.x < 64) s_sum[threadIdx.x] = 0.0;
+  __syncthreads();
   const int slot = (*itp) & 1;
-  double v = 0.0;
-  int f = 0;
-  if (F == 1) {
-    if (idx < n) {
-      float dvs = s[idx];
-      dvs += (float)(dvs == 0.0f);
-      float xn = x[idx];
-      if (!active || active[0]) { xn = x[idx] * c[idx] / dvs; x[idx] = xn; }
-      v = xn;
-    }
-    const double t = block_sum_f64(v, s_w);
-    if (threadIdx.x == 0) unsafeAtomicAdd(&sum[slot], t);
-  } else {
-    if (idx < n * F) {
-      const long long j = idx / F;
-      f = (int)(idx % F);
-      float dvs = s[j];
-      dvs += (float)(dvs == 0.0f);
-      float xn = x[idx];
-      if (!active || active[f]) { xn = x[idx] * c[idx] / dvs; x[idx] = xn; }
-      unsafeAtomicAdd(&sum[slot * F + f], (double)xn);
-    }
+  const long long tot = n * F;
+  double v1 = 0.0;   // single-frame running sum
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long j = (F == 1) ? idx : idx / F;
+    const int f = (F == 1) ? 0 : (int)(idx % F);
+    float dvs = s[j];
+    dvs += (float)(dvs == 0.0f);   // mlemOperations.hpp:48-50
+    float xn = x[idx];
+    if (!active || active[f]) { xn = xn * c[idx] / dvs; x[idx] = xn; }
+    if (F == 1) v1 += xn;
+    else atomicAdd(&s_sum[f], (double)xn);
   }
+  if (F == 1) {
+    v1 = wave_sum(v1);
+    if (lane_id() == 0) atomicAdd(&s_sum[0], v1);
+  }
+  __syncthreads();
+  if (threadIdx.x < F) unsafeAtomicAdd(&sum[slot * F + threadIdx.x], s_sum[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ x, float* __restrict__ c,
@@ -304,23 +414,63 @@ static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F
   return EPET_OK;
 }
 
+template <typename VT, int P>
+static void mf2_kernels(epet_ctx* ctx, const epet_matrix* A, int F, const float* x, const float* y,
+                        float* e, float* c, const int* itp, double* loglik) {
+  using T = typename VT::T;
+  hipStream_t st = ctx->stream;
+  const long long cap = 16ll * ctx->num_cus;
+  const unsigned g1 = (unsigned)std::max<long long>(NXCD, std::min<long long>(((long long)A->m + 3) / 4, cap));
+  k_fwd_mf2<VT, P><<<g1, 256, 0, st>>>(A->m, F, A->rowptr, A->col, (const T*)A->val, A->rowscale, x,
+                                       y, e, A->rowscale ? 1 : 0, itp, loglik);
+  const epet_matrix* At = A->tr;
+  const unsigned g2 = (unsigned)std::max<long long>(NXCD, std::min<long long>(((long long)At->m + 3) / 4, cap));
+  k_bp_gather_mf<VT, P><<<g2, 256, 0, st>>>(At->m, F, At->rowptr, At->col, (const T*)At->val, e, c);
+}
+
+template <typename VT>
+static int launch_mf2_vt(epet_ctx* ctx, const epet_matrix* A, int F, const float* x,
+                         const float* y, float* e, float* c, const int* itp, double* loglik) {
+  if (F <= 2) mf2_kernels<VT, 2>(ctx, A, F, x, y, e, c, itp, loglik);
+  else if (F <= 4) mf2_kernels<VT, 4>(ctx, A, F, x, y, e, c, itp, loglik);
+  else if (F <= 8) mf2_kernels<VT, 8>(ctx, A, F, x, y, e, c, itp, loglik);
+  else if (F <= 16) mf2_kernels<VT, 16>(ctx, A, F, x, y, e, c, itp, loglik);
+  else if (F <= 32) mf2_kernels<VT, 32>(ctx, A, F, x, y, e, c, itp, loglik);
+  else mf2_kernels<VT, 64>(ctx, A, F, x, y, e, c, itp, loglik);
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+static int launch_mf2(epet_ctx* ctx, const epet_matrix* A, int F, const float* x, const float* y,
+                      float* e, float* c, const int* itp, double* loglik) {
+  switch (A->value_type) {
+    case EPET_VAL_F32: return launch_mf2_vt<ValF32>(ctx, A, F, x, y, e, c, itp, loglik);
+    case EPET_VAL_BF16: return launch_mf2_vt<ValBF16>(ctx, A, F, x, y, e, c, itp, loglik);
+    default: return launch_mf2_vt<ValF16>(ctx, A, F, x, y, e, c, itp, loglik);
+  }
+}
+
 // One MLEM iteration on `st` (capturable).
 static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const float* y,
                           const float* s, float* x, float* e, float* c, int* itp, double* sum,
                           double* loglik, const int* active) {
   hipStream_t st = ctx->stream;
   int rc;
-  switch (A->value_type) {
-    case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
-    case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
-    default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+  if (F > 1 && A->tr) {
+    rc = launch_mf2(ctx, A, F, x, y, e, c, itp, loglik);
+  } else {
+    switch (A->value_type) {
+      case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+      case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+      default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+    }
+    if (rc != EPET_OK) return rc;
+    if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
+    else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
   }
   if (rc != EPET_OK) return rc;
-  if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
-  else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
-  if (rc != EPET_OK) return rc;
   const long long nv = (long long)A->n * F;
-  k_update<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(x, c, s, A->n, F, itp, sum, active);
+  k_update<<<grid_for(nv, 256, 4ll * ctx->num_cus), 256, 0, st>>>(x, c, s, A->n, F, itp, sum, active);
   k_normalise<<<grid_for(std::max<long long>(nv, F), 256, 1ll << 31), 256, 0, st>>>(x, c, A->n, F,
                                                                                    itp, sum, active);
   k_advance<<<1, 64, 0, st>>>(itp);

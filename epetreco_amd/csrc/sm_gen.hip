@@ -508,6 +508,15 @@ int epet_getWorkqueue(epet_ctx* ctx, const int* list_dev, int list_size, int* li
   int rc = exclusive_scan_i32_to_i64(ctx, counts, offs, rows, &total);
   if (rc != EPET_OK) return rc;
   const long long want = cap < 0 ? total : std::min(cap, total);
+  if (!out_cnl_dev && !out_vxl_dev && out_capacity == 0) {
+    // count-only query: the number of pairs a call with this cap would emit;
+    // the cursor is left unchanged
+    (void)hipFreeAsync(counts, ctx->stream);
+    (void)hipFreeAsync(offs, ctx->stream);
+    EPET_HIP(hipStreamSynchronize(ctx->stream));
+    *n_found = want;
+    return EPET_OK;
+  }
   if (want > out_capacity || (want > 0 && (!out_cnl_dev || !out_vxl_dev))) {
     (void)hipFreeAsync(counts, ctx->stream);
     (void)hipFreeAsync(offs, ctx->stream);

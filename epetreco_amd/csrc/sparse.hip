@@ -288,9 +288,69 @@ __global__ __launch_bounds__(256) void k_check_csr(int m, int n, long long nnz,
 
 using namespace epet;
 
+namespace epet {
+// ------------------------------------------------------------------ transpose
+// Deterministic transpose without atomics: workgroup b walks its block of rows
+// in order, one row at a time (columns are distinct within a row, a barrier
+// separates rows), counting entries per (column, block) and later placing them
+// at column offset + block offset + running count.  Entries of a column come
+// out in ascending row order.
+constexpr int TR_BLOCK = 256;
+
+__global__ __launch_bounds__(TR_BLOCK) void k_tr_count(int m, int rows_per_block, int B,
+                                                       const int64_t* __restrict__ rowptr,
+                                                       const int* __restrict__ col,
+                                                       int* __restrict__ cnt) {
+  const int blk = blockIdx.x;
+  const long long r0 = (long long)blk * rows_per_block;
+  const long long r1 = min((long long)m, r0 + rows_per_block);
+  for (long long r = r0; r < r1; r++) {
+    const int64_t b = rowptr[r], e = rowptr[r + 1];
+    for (int64_t k = b + threadIdx.x; k < e; k += TR_BLOCK) {
+      int* p = &cnt[(size_t)col[k] * B + blk];
+      *p = *p + 1;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(TR_BLOCK) void k_tr_fill(int m, int rows_per_block, int B,
+                                                      const int64_t* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const V* __restrict__ val,
+                                                      int64_t* __restrict__ cursor,
+                                                      int* __restrict__ orow,
+                                                      V* __restrict__ oval) {
+  const int blk = blockIdx.x;
+  const long long r0 = (long long)blk * rows_per_block;
+  const long long r1 = min((long long)m, r0 + rows_per_block);
+  for (long long r = r0; r < r1; r++) {
+    const int64_t b = rowptr[r], e = rowptr[r + 1];
+    for (int64_t k = b + threadIdx.x; k < e; k += TR_BLOCK) {
+      int64_t* p = &cursor[(size_t)col[k] * B + blk];
+      const int64_t pos = *p;
+      *p = pos + 1;
+      orow[pos] = (int)r;
+      oval[pos] = val[k];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tr_rowptr(int n, int B, long long nnz,
+                                                   const int64_t* __restrict__ off,
+                                                   int64_t* __restrict__ tptr) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) tptr[j] = off[(size_t)j * B];
+  if (j == n) tptr[n] = nnz;
+}
+}  // namespace epet
+
 epet_matrix::~epet_matrix() {
   if (bp_toff) (void)hipFree(bp_toff);
   if (bp_partial) (void)hipFree(bp_partial);
+  if (tr) epet_matrix_destroy(tr);
 }
 
 namespace epet {
@@ -458,7 +518,7 @@ int epet_matrix_create(epet_ctx* ctx, int m, int n, long long nnz, const int64_t
   epet_matrix* A = nullptr;
   EPET_TRY(matrix_alloc(&A, m, n, nnz, EPET_VAL_F32));
   // the tiled back projection needs ascending columns within rows
-  if (hflags & 2) A->bp_failed = true;
+  if (hflags & 2) { A->bp_failed = true; A->cols_strict = false; }
   EPET_HIP(hipMemcpyAsync(A->rowptr, rowptr, sizeof(int64_t) * ((size_t)m + 1),
                           hipMemcpyDeviceToDevice, st));
   if (nnz) {
@@ -556,6 +616,49 @@ int epet_matrix_colsum_f64(epet_ctx* ctx, const epet_matrix* A, double* s) {
     k_colsum_f64<<<A->m, 256, 0, st>>>(A->rowptr, A->col, A->val, A->value_type, A->rowscale,
                                        A->m, s);
   EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A) {
+  EPET_CHECK(ctx && A, EPET_ERR_ARG, "null argument");
+  if (A->tr) return EPET_OK;
+  EPET_CHECK(A->cols_strict, EPET_ERR_STATE,
+             "transpose needs strictly ascending columns within every row");
+  hipStream_t st = ctx->stream;
+  const int rows_per_block = 4096;
+  const int B = std::max(1, (A->m + rows_per_block - 1) / rows_per_block);
+  const long long ncnt = (long long)A->n * B;
+  int* cnt = nullptr;
+  int64_t* off = nullptr;
+  EPET_HIP(hipMallocAsync((void**)&cnt, sizeof(int) * std::max(1ll, ncnt), st));
+  EPET_HIP(hipMallocAsync((void**)&off, sizeof(int64_t) * (ncnt + 1), st));
+  k_fill_i32<<<grid_for(ncnt, 256, 1ll << 31), 256, 0, st>>>(cnt, ncnt, 0);
+  if (A->m) k_tr_count<<<B, TR_BLOCK, 0, st>>>(A->m, rows_per_block, B, A->rowptr, A->col, cnt);
+  EPET_HIP(hipGetLastError());
+  long long tot = 0;
+  EPET_TRY(exclusive_scan_i32_to_i64(ctx, cnt, off, ncnt, &tot));
+  EPET_CHECK(tot == A->nnz, EPET_ERR_STATE, "transpose count mismatch");
+  epet_matrix* T = nullptr;
+  EPET_TRY(matrix_alloc(&T, A->n, A->m, A->nnz, A->value_type == EPET_VAL_F32 ? EPET_VAL_F32
+                                                                              : A->value_type));
+  if (T->rowscale) { (void)hipFree(T->rowscale); T->rowscale = nullptr; }
+  k_tr_rowptr<<<grid_for((long long)A->n + 1, 256, 1ll << 31), 256, 0, st>>>(A->n, B, A->nnz, off,
+                                                                           T->rowptr);
+  if (A->m) {
+    if (A->value_type == EPET_VAL_F32)
+      k_tr_fill<float><<<B, TR_BLOCK, 0, st>>>(A->m, rows_per_block, B, A->rowptr, A->col,
+                                               (const float*)A->val, off, T->col, (float*)T->val);
+    else
+      k_tr_fill<uint16_t><<<B, TR_BLOCK, 0, st>>>(A->m, rows_per_block, B, A->rowptr, A->col,
+                                                  (const uint16_t*)A->val, off, T->col,
+                                                  (uint16_t*)T->val);
+  }
+  EPET_HIP(hipGetLastError());
+  (void)hipFreeAsync(cnt, st);
+  (void)hipFreeAsync(off, st);
+  EPET_HIP(hipStreamSynchronize(st));
+  T->bp_failed = true;
+  A->tr = T;
   return EPET_OK;
 }
 

@@ -107,7 +107,9 @@ int epet_rays(epet_ctx* ctx, const int* cnl_dev, int n, float* rays_dev);
 /* Scans (list entry, voxel) pairs from the cursor (*listId, *vxlId) in
  * lexicographic order and emits candidate pairs (getWorkqueue.hpp:50-78),
  * at most `cap` of them (cap < 0: all; then out_capacity must hold them).
- * Advances the cursor exactly as the reference.  *n_found = pairs written. */
+ * Advances the cursor exactly as the reference.  *n_found = pairs written.
+ * With both output pointers NULL and out_capacity 0 it only counts: *n_found =
+ * the number of pairs the call would emit, cursor unchanged. */
 int epet_getWorkqueue(epet_ctx* ctx, const int* list_dev, int list_size,
                       int* listId, int* vxlId, long long cap,
                       int* out_cnl_dev, int* out_vxl_dev, long long out_capacity,
@@ -170,6 +172,11 @@ int epet_matrix_select_rows(epet_ctx* ctx, const epet_matrix* A, const int* rows
  * per-row scale = row max, stored value = a / scale rounded to nearest even). */
 int epet_matrix_convert(epet_ctx* ctx, const epet_matrix* A, int value_type,
                         epet_matrix** out);
+/* Attach a transposed copy of A (rows = voxels, entries in ascending row
+ * order, built deterministically without atomics); the multi-frame back
+ * projection of epet_mlem then gathers from it.  Needs strictly ascending
+ * columns within rows. */
+int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A);
 /* Column sums in FP64 (the sensitivity image s_j = sum_i a_ij). */
 int epet_matrix_colsum_f64(epet_ctx* ctx, const epet_matrix* A, double* s_dev);
 
