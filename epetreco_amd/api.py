@@ -220,6 +220,16 @@ class Context:
         check(lib().epet_sum(self._h, _p(x), x.numel(), ctypes.byref(out)))
         return out.value
 
+    def mlem_tune(self, A: "Matrix", frames: int):
+        """Autotune the multi-frame kernels for `frames`; returns
+        ((R_fwd, P_fwd, R_bwd, P_bwd), (ms_fwd, ms_bwd))."""
+        cfg = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(2, dtype=np.float32)
+        check(lib().epet_mlem_tune(self._h, A.handle, int(frames),
+                                   cfg.ctypes.data_as(_lib.c_int_p),
+                                   ms.ctypes.data_as(_lib.c_float_p)))
+        return tuple(int(v) for v in cfg), tuple(float(v) for v in ms)
+
     def mlem(self, A: "Matrix", y: torch.Tensor, s: torch.Tensor, x: torch.Tensor,
              niter: int, use_graph: bool = False, tol: float = 0.0):
         """In-place MLEM on x (n x F, frame-minor); returns (loglik[niter, F], iters[F])."""

@@ -31,6 +31,7 @@ SOURCES = {
     "sparse.hip": [],
     "spmv.hip": [],
     "mlem.hip": [],
+    "mframe.hip": [],
 }
 
 

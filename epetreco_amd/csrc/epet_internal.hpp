@@ -97,6 +97,10 @@ struct epet_matrix {
   // used by the multi-frame back projection; per-row scales of A are folded
   // into the back-projected vector instead
   epet_matrix* tr = nullptr;
+  // multi-frame kernel configuration picked by the autotuner for mf_F frames
+  // (rows per wavefront R, frames per lane tile P) and the times it measured
+  int mf_F = 0, mf_fwd_R = 1, mf_fwd_P = 32, mf_bwd_R = 1, mf_bwd_P = 32;
+  float mf_fwd_ms = 0.0f, mf_bwd_ms = 0.0f;
   ~epet_matrix();
 };
 
@@ -117,6 +121,16 @@ int exclusive_scan_i64(epet_ctx* ctx, const int64_t* in, int64_t* out, long long
                        long long* total);
 int exclusive_scan_i32(epet_ctx* ctx, const int* in, int* out, long long n,
                        long long* total);
+
+// ---------------------------------------------------------------- multi-frame
+// Buffers of one multi-frame MLEM iteration (frame-minor, F frames).
+struct MfArgs {
+  int F;
+  const float *x, *y;
+  float *e, *c;
+  const int* itp;
+  double* loglik;
+};
 
 // ---------------------------------------------------------------- launch helpers
 inline unsigned grid_for(long long n, int block, long long max_blocks = 1 << 20) {

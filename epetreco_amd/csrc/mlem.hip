@@ -29,6 +29,8 @@ int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha,
                 float beta, float* Y);
 int pick_lanes(long long nnz, int m);
 int bp_prepare(epet_ctx* ctx, epet_matrix* A);
+int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
+int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
 
 // ------------------------------------------------------------------ element-wise
 __global__ __launch_bounds__(256) void k_divides(float* __restrict__ res, const float* __restrict__ a,
@@ -193,117 +195,6 @@ __global__ __launch_bounds__(256) void k_fwd_ratio_mf(int m, int F,
   }
 }
 
-// ------------------------------------------------------------------ multi-frame v2
-// XCD-aware persistent mapping: workgroups are dispatched round-robin over the
-// 8 XCDs, so workgroup b works on the contiguous eighth (b mod 8) of the rows;
-// concurrently processed rows on one XCD are then neighbours in channel order
-// and share image rows in that XCD's L2.  (Only locality depends on this.)
-constexpr int NXCD = 8;
-
-__device__ __forceinline__ void xcd_range(long long n, long long& beg, long long& end,
-                                          long long& first, long long& stride) {
-  const int xcd = blockIdx.x % NXCD;
-  const int kb = blockIdx.x / NXCD;
-  const int nb = (gridDim.x - xcd + NXCD - 1) / NXCD;   // workgroups on this XCD
-  const long long chunk = (n + NXCD - 1) / NXCD;
-  beg = min(n, xcd * chunk);
-  end = min(n, beg + chunk);
-  first = beg + (long long)kb * (256 / WAVE) + wave_id();
-  stride = (long long)nb * (256 / WAVE);
-}
-
-// Forward for F frames (frame-minor X), one wavefront per row: lanes load 64
-// (column, value) pairs at once, then lane (slot s, frame f) gathers row
-// X[col] of 64/P entries per step.  Epilogue: E = Y / Yhat (0 where Yhat = 0),
-// times the row scale when fold = 1 (the transposed back projection has no
-// row scales), and the FP64 log-likelihood per frame.
-template <typename VT, int P>
-__global__ __launch_bounds__(256) void k_fwd_mf2(int m, int F, const int64_t* __restrict__ rowptr,
-                                                 const int* __restrict__ col,
-                                                 const typename VT::T* __restrict__ val,
-                                                 const float* __restrict__ rowscale,
-                                                 const float* __restrict__ X,
-                                                 const float* __restrict__ Y,
-                                                 float* __restrict__ E, int fold,
-                                                 const int* __restrict__ itp,
-                                                 double* __restrict__ loglik) {
-  constexpr int S = WAVE / P;
-  __shared__ double s_ll[256 / WAVE][WAVE];
-  const int lane = lane_id();
-  const int f = lane % P, s = lane / P;
-  long long beg, end, r, stride;
-  xcd_range(m, beg, end, r, stride);
-  double ll = 0.0;
-  for (; r < end; r += stride) {
-    const long long b = rowptr[r], e = rowptr[r + 1];
-    float acc = 0.0f;
-    for (long long base = b; base < e; base += WAVE) {
-      const long long k = base + lane;
-      const int ci = k < e ? ldcol(col, k) : 0;
-      const float cv = k < e ? VT::get(val, k) : 0.0f;
-      const int nn = (int)min((long long)WAVE, e - base);
-#pragma unroll 4
-      for (int q = 0; q < nn; q += S) {
-        const int c = __shfl(ci, q + s, WAVE);
-        const float v = __shfl(cv, q + s, WAVE);
-        if (f < F) acc += v * X[(long long)c * F + f];
-      }
-    }
-#pragma unroll
-    for (int off = WAVE / 2; off >= P; off >>= 1) acc += __shfl_xor(acc, off, WAVE);
-    if (s == 0 && f < F) {
-      const float sc = rowscale ? rowscale[r] : 1.0f;
-      const float yh = acc * sc;
-      const float yi = Y[r * F + f];
-      const float ev = yh > 0.0f ? yi / yh : 0.0f;
-      E[r * F + f] = fold ? ev * sc : ev;
-      if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
-    }
-  }
-  s_ll[wave_id()][lane] = (s == 0) ? ll : 0.0;
-  __syncthreads();
-  if (loglik && threadIdx.x < F) {
-    double t = 0.0;
-    for (int w = 0; w < 256 / WAVE; w++) t += s_ll[w][threadIdx.x];
-    unsafeAtomicAdd(&loglik[(long long)(*itp) * F + threadIdx.x], t);
-  }
-}
-
-// Back projection for F frames from the transposed matrix: one wavefront per
-// voxel j, C[j][f] = sum over the voxel's entries (i, v) of v * E[i][f].
-template <typename VT, int P>
-__global__ __launch_bounds__(256) void k_bp_gather_mf(int n, int F,
-                                                      const int64_t* __restrict__ tptr,
-                                                      const int* __restrict__ trow,
-                                                      const typename VT::T* __restrict__ tval,
-                                                      const float* __restrict__ E,
-                                                      float* __restrict__ C) {
-  constexpr int S = WAVE / P;
-  const int lane = lane_id();
-  const int f = lane % P, s = lane / P;
-  long long beg, end, j, stride;
-  xcd_range(n, beg, end, j, stride);
-  for (; j < end; j += stride) {
-    const long long b = tptr[j], e = tptr[j + 1];
-    float acc = 0.0f;
-    for (long long base = b; base < e; base += WAVE) {
-      const long long k = base + lane;
-      const int ri = k < e ? ldcol(trow, k) : 0;
-      const float rv = k < e ? VT::get(tval, k) : 0.0f;
-      const int nn = (int)min((long long)WAVE, e - base);
-#pragma unroll 4
-      for (int q = 0; q < nn; q += S) {
-        const int i = __shfl(ri, q + s, WAVE);
-        const float v = __shfl(rv, q + s, WAVE);
-        if (f < F) acc += v * E[(long long)i * F + f];
-      }
-    }
-#pragma unroll
-    for (int off = WAVE / 2; off >= P; off >>= 1) acc += __shfl_xor(acc, off, WAVE);
-    if (s == 0 && f < F) C[j * F + f] = acc;
-  }
-}
-
 // x <- x c / (s + [s==0]) for active frames, FP64 sum per frame into slot it&1
 // (grid-stride; per-frame partial sums in LDS, one global atomic per
 // workgroup and frame)
@@ -414,42 +305,6 @@ static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F
   return EPET_OK;
 }
 
-template <typename VT, int P>
-static void mf2_kernels(epet_ctx* ctx, const epet_matrix* A, int F, const float* x, const float* y,
-                        float* e, float* c, const int* itp, double* loglik) {
-  using T = typename VT::T;
-  hipStream_t st = ctx->stream;
-  const long long cap = 16ll * ctx->num_cus;
-  const unsigned g1 = (unsigned)std::max<long long>(NXCD, std::min<long long>(((long long)A->m + 3) / 4, cap));
-  k_fwd_mf2<VT, P><<<g1, 256, 0, st>>>(A->m, F, A->rowptr, A->col, (const T*)A->val, A->rowscale, x,
-                                       y, e, A->rowscale ? 1 : 0, itp, loglik);
-  const epet_matrix* At = A->tr;
-  const unsigned g2 = (unsigned)std::max<long long>(NXCD, std::min<long long>(((long long)At->m + 3) / 4, cap));
-  k_bp_gather_mf<VT, P><<<g2, 256, 0, st>>>(At->m, F, At->rowptr, At->col, (const T*)At->val, e, c);
-}
-
-template <typename VT>
-static int launch_mf2_vt(epet_ctx* ctx, const epet_matrix* A, int F, const float* x,
-                         const float* y, float* e, float* c, const int* itp, double* loglik) {
-  if (F <= 2) mf2_kernels<VT, 2>(ctx, A, F, x, y, e, c, itp, loglik);
-  else if (F <= 4) mf2_kernels<VT, 4>(ctx, A, F, x, y, e, c, itp, loglik);
-  else if (F <= 8) mf2_kernels<VT, 8>(ctx, A, F, x, y, e, c, itp, loglik);
-  else if (F <= 16) mf2_kernels<VT, 16>(ctx, A, F, x, y, e, c, itp, loglik);
-  else if (F <= 32) mf2_kernels<VT, 32>(ctx, A, F, x, y, e, c, itp, loglik);
-  else mf2_kernels<VT, 64>(ctx, A, F, x, y, e, c, itp, loglik);
-  EPET_HIP(hipGetLastError());
-  return EPET_OK;
-}
-
-static int launch_mf2(epet_ctx* ctx, const epet_matrix* A, int F, const float* x, const float* y,
-                      float* e, float* c, const int* itp, double* loglik) {
-  switch (A->value_type) {
-    case EPET_VAL_F32: return launch_mf2_vt<ValF32>(ctx, A, F, x, y, e, c, itp, loglik);
-    case EPET_VAL_BF16: return launch_mf2_vt<ValBF16>(ctx, A, F, x, y, e, c, itp, loglik);
-    default: return launch_mf2_vt<ValF16>(ctx, A, F, x, y, e, c, itp, loglik);
-  }
-}
-
 // One MLEM iteration on `st` (capturable).
 static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const float* y,
                           const float* s, float* x, float* e, float* c, int* itp, double* sum,
@@ -457,7 +312,8 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
   hipStream_t st = ctx->stream;
   int rc;
   if (F > 1 && A->tr) {
-    rc = launch_mf2(ctx, A, F, x, y, e, c, itp, loglik);
+    const MfArgs a{F, x, y, e, c, itp, loglik};
+    rc = mf_iteration(ctx, A, a);
   } else {
     switch (A->value_type) {
       case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
@@ -580,10 +436,15 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   EPET_MLEM_HIP(hipGetLastError());
   EPET_MLEM_HIP(hipStreamSynchronize(st));
 
-  // back-projection plan (allocates) before any graph capture
+  // back-projection plan (allocates) and multi-frame tuning before any capture
   if (F == 1) {
     rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }
+  } else if (A->tr && A->mf_F != F) {
+    const MfArgs a{F, x, y, e, c, itp, nullptr};
+    rc = mf_tune(ctx, const_cast<epet_matrix*>(A), a, 2);
+    if (rc != EPET_OK) { cleanup(); return rc; }
+    k_fill_f32<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(c, nv, 0.0f);
   }
   const bool stopping = tol > 0.0 && frame_iters != nullptr;
   std::vector<int> act(F, 1), iters(F, 0);
@@ -647,6 +508,35 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   }
   cleanup();
 #undef EPET_MLEM_HIP
+  return rc;
+}
+
+int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms) {
+  EPET_CHECK(ctx && A, EPET_ERR_ARG, "null argument");
+  EPET_CHECK(F >= 2 && F <= 64, EPET_ERR_UNSUPPORTED, "frames must be in [2, 64]");
+  EPET_CHECK(A->tr, EPET_ERR_STATE, "attach the transpose first (epet_matrix_transpose)");
+  hipStream_t st = ctx->stream;
+  float *x = nullptr, *y = nullptr, *e = nullptr, *c = nullptr;
+  int* itp = nullptr;
+  const long long nx = (long long)A->n * F, ny = (long long)A->m * F;
+  EPET_HIP(hipMallocAsync((void**)&x, sizeof(float) * std::max(1ll, nx), st));
+  EPET_HIP(hipMallocAsync((void**)&c, sizeof(float) * std::max(1ll, nx), st));
+  EPET_HIP(hipMallocAsync((void**)&y, sizeof(float) * std::max(1ll, ny), st));
+  EPET_HIP(hipMallocAsync((void**)&e, sizeof(float) * std::max(1ll, ny), st));
+  EPET_HIP(hipMallocAsync((void**)&itp, sizeof(int), st));
+  k_fill_f32<<<grid_for(nx, 256, 1ll << 31), 256, 0, st>>>(x, nx, 1.0f);
+  k_fill_f32<<<grid_for(ny, 256, 1ll << 31), 256, 0, st>>>(y, ny, 1.0f);
+  k_advance<<<1, 64, 0, st>>>(itp);
+  const MfArgs a{F, x, y, e, c, itp, nullptr};
+  A->mf_F = 0;
+  int rc = mf_tune(ctx, A, a, 3);
+  (void)hipFreeAsync(x, st); (void)hipFreeAsync(y, st); (void)hipFreeAsync(e, st);
+  (void)hipFreeAsync(c, st); (void)hipFreeAsync(itp, st);
+  EPET_HIP(hipStreamSynchronize(st));
+  if (rc == EPET_OK) {
+    if (cfg) { cfg[0] = A->mf_fwd_R; cfg[1] = A->mf_fwd_P; cfg[2] = A->mf_bwd_R; cfg[3] = A->mf_bwd_P; }
+    if (ms) { ms[0] = A->mf_fwd_ms; ms[1] = A->mf_bwd_ms; }
+  }
   return rc;
 }
 

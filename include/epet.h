@@ -233,6 +233,13 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int nframes, const float* y_d
               const float* s_dev, float* x_dev, int niter, int use_graph,
               double tol, double* loglik_out, int* frame_iters);
 
+/* Autotuner of the multi-frame kernels: times every (rows per wavefront R in
+ * {1,2,4,8}) x (frames per lane tile P in {4,8,16,32}) pair on A and its
+ * attached transpose for F frames and keeps the fastest forward and back
+ * projection.  cfg (4 ints, may be NULL) = {R_fwd, P_fwd, R_bwd, P_bwd}; ms (2
+ * floats, may be NULL) = their times.  epet_mlem runs it on first use. */
+int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms);
+
 /* ---- chunk helpers (wrappers.hpp:405-445) ---------------------------------------- */
 uint32_t epet_nInChunk(uint32_t idChunk, uint32_t nAll, uint32_t nFullChunk);
 uint32_t epet_chunkPtr(uint32_t chunkId, uint32_t nFullChunk);

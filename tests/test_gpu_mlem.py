@@ -128,3 +128,39 @@ def test_mlem_batched_transpose_path(small_problem, oracle, storage):
                               np.full(A.n, 1.0 / A.n), 10)
         assert rel_l2(X[:, f].cpu().numpy(), x64) < tol
         assert np.all(np.diff(ll[:, f]) >= -1e-6 * np.abs(ll[:-1, f]))
+
+
+def test_graph_replays_match_eager(small_problem):
+    """Every replay of the captured iteration equals the eager iteration: the
+    per-iteration log-likelihoods (one per replay) and the final image."""
+    ctx, prob = small_problem
+    A = prob.A
+    xg = torch.full((A.n,), 1.0 / A.n, device="cuda")
+    xe = xg.clone()
+    llg, _ = ctx.mlem(A, prob.y, prob.s, xg, 12, use_graph=True)
+    torch.cuda.synchronize()
+    _ = (xg * 2).sum().item()          # eager work between the runs
+    lle, _ = ctx.mlem(A, prob.y, prob.s, xe, 12, use_graph=False)
+    # LDS float atomics make the summation order vary between runs
+    np.testing.assert_allclose(llg[:, 0], lle[:, 0], rtol=1e-9)
+    assert rel_l2(xg.cpu().numpy(), xe.cpu().numpy().astype(np.float64)) < 1e-5
+
+
+def test_multiframe_autotuner(small_problem):
+    """The tuner times every (rows per wavefront, frame tile) pair and keeps
+    valid choices; results with the tuned kernels match the default ones."""
+    ctx, prob = small_problem
+    from epetreco_amd.api import Matrix
+    rp, col, val, _ = prob.A.to_torch()
+    A = Matrix.from_csr(ctx, rp, col, val, prob.A.n).attach_transpose()
+    (Rf, Pf, Rb, Pb), (mf, mb) = ctx.mlem_tune(A, 16)
+    _say("tuned fwd R,P =", Rf, Pf, f"{mf:.3f} ms; bwd R,P =", Rb, Pb, f"{mb:.3f} ms")
+    assert Rf in (1, 2, 4, 8) and Pf in (4, 8, 16, 32) and Pf <= 64 // Rf
+    assert Rb in (1, 2, 4, 8) and Pb in (4, 8, 16, 32) and Pb <= 64 // Rb
+    Y = torch.rand(A.m, 16, device="cuda") * 5
+    X1 = torch.full((A.n, 16), 1.0 / A.n, device="cuda")
+    X2 = X1.clone()
+    ctx.mlem(A, Y, prob.s, X1, 3)                  # tuned configuration
+    B = Matrix.from_csr(ctx, rp, col, val, prob.A.n)   # atomic back projection
+    ctx.mlem(B, Y, prob.s, X2, 3)
+    assert rel_l2(X1.cpu().numpy(), X2.cpu().numpy()) < 1e-5
