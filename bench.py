@@ -84,6 +84,27 @@ def cpu_baseline(A, y_dev, s_dev, target_s=15.0):
                       f"median of {len(times)}, scaled by nonzeros"}
 
 
+def spmv_bytes(nnz, m, V):
+    """Algorithmic bytes of one FP32-ECSR SpMV: values and columns once, int64
+    row pointers, the row-side vector once and the voxel-side vector once."""
+    return nnz * 8 + (m + 1) * 8 + m * 4 + V * 4
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading
+    for a device copy of known size), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    return k["fetch_bytes"] * t["copy_scale_fetch"] + k["write_bytes"] * t["copy_scale_write"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +115,9 @@ def main():
     ap.add_argument("--workload", default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="after building the problem, launch only the forward SpMV, the back "
+                         "projection and a 1 GiB device copy (3 times each) for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X; "
@@ -129,6 +153,25 @@ def main():
     m = A.m
     say(f"measured channels {m}, nonzeros {A.nnz}")
 
+    if args.profile_kernels:
+        xv = torch.rand(V, device="cuda")
+        ev = torch.rand(m, device="cuda")
+        yv = torch.zeros(m, device="cuda")
+        cv = torch.zeros(V, device="cuda")
+        for _ in range(3):
+            A.spmv(OP_N, xv, yv)
+        for _ in range(3):
+            A.spmv(OP_T, ev, cv, beta=1.0)
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)       # elementwise kernel: 1 GiB read, 1 GiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernels": True, "nnz": A.nnz, "rows": m, "voxels": V,
+                          "algorithmic_bytes_fwd": spmv_bytes(A.nnz, m, V),
+                          "copy_bytes_each_way": src.numel() * 4}))
+        return
+
     # ---- plain FP32 MLEM: the timed steps
     x0 = torch.full((V,), 1.0 / V, device="cuda")
     x = x0.clone()
@@ -153,8 +196,8 @@ def main():
     A.spmv(OP_N, xv, yv)
     fwd_ms = event_time_ms(lambda: A.spmv(OP_N, xv, yv), 10)
     bwd_ms = event_time_ms(lambda: A.spmv(OP_T, ev, cv, beta=1.0), 5)
-    B_fwd = A.nnz * 8 + (m + 1) * 8 + m * 4 + V * 4
-    B_bwd = A.nnz * 8 + (m + 1) * 8 + m * 4 + V * 4
+    B_fwd = spmv_bytes(A.nnz, m, V)
+    B_bwd = spmv_bytes(A.nnz, m, V)
     fwd_gbps = B_fwd / fwd_ms / 1e6
     bwd_gbps = B_bwd / bwd_ms / 1e6
     say(f"forward SpMV {fwd_ms:.3f} ms = {fwd_gbps:.0f} GB/s; back projection {bwd_ms:.3f} ms "
@@ -241,8 +284,9 @@ def main():
         "generation_s": gen_s,
         "batched": batched,
         "roofline": {"bound": "hbm", "achieved": fwd_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": fwd_gbps / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "forward SpMV (k_spmv_n)",
+                     "frac": fwd_gbps / HBM_PEAK_GBPS,
+                     "traffic": measured_traffic(args.workload, "k_spmv_n_vec"),
+                     "kernel": "forward SpMV (k_spmv_n_vec)",
                      "algorithmic_bytes": B_fwd, "avg_ms": fwd_ms},
         "cpu_baseline": cpu,
     }

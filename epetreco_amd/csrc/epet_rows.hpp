@@ -30,6 +30,19 @@ struct ValF16 {
   }
 };
 
+// Row bounds prefetch: lanes 0 and 1 load rowptr[r] and rowptr[r + 1] in one
+// vector load (vector loads complete in order, so waiting for the row in
+// flight never waits for this younger request); row_bounds() reads them out.
+__device__ __forceinline__ long long row_bounds_load(const int64_t* __restrict__ rowptr,
+                                                     long long r, int lane) {
+  return rowptr[r + (lane & 1)];
+}
+__device__ __forceinline__ long long lane_value_i64(long long v, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)v, l);
+  const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)v >> 32), l);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 template <typename IDX>
 __device__ __forceinline__ int ldcol(const int* p, IDX k) {
   return __builtin_nontemporal_load(p + k);
@@ -96,6 +109,53 @@ __device__ __forceinline__ float row_dot_wave(long long b, long long e, int lane
   const long long kt = kh + (nq << 2);
   if (kt + lane < e) acc1 += VT::get(val, kt + lane) * x[ldcol(col, kt + lane)];
   return acc0 + acc1;
+}
+
+// Same sum with every load of a pass issued before any value is used: U
+// quads per lane (256U entries per wavefront per pass) plus, on the first
+// pass, the unaligned head and the tail.  Out-of-range slots load a clamped
+// in-row address and are zeroed by a select at use, so no load sits behind a
+// branch and a row of up to 256U entries costs one HBM and one L2 round trip.
+// Requires b < e.
+template <typename VT, int U>
+__device__ __forceinline__ float row_dot_wave_u(long long b, long long e, int lane,
+                                                const int* __restrict__ col,
+                                                const typename VT::T* __restrict__ val,
+                                                const float* __restrict__ x) {
+  long long kh = (b + 3) & ~3ll;
+  if (kh > e) kh = e;
+  const long long nq = (e - kh) >> 2;
+  const long long kt = kh + (nq << 2);
+  const long long last = e - 1;
+  // head and tail entries of this lane (one each at most)
+  const bool hv = b + lane < kh, tv = kt + lane < e;
+  const long long hk = hv ? b + lane : last, tk = tv ? kt + lane : last;
+  const int hc = ldcol(col, hk), tc = ldcol(col, tk);
+  const float ha = VT::get(val, hk), ta = VT::get(val, tk);
+  float acc = 0.0f;
+  for (long long q0 = 0; q0 < nq; q0 += (long long)U * WAVE) {
+    i32x4 c[U];
+    float v[U][4];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long long q = q0 + lane + (long long)u * WAVE;
+      ok[u] = q < nq;
+      const long long k = kh + 4 * (ok[u] ? q : nq - 1);
+      c[u] = __builtin_nontemporal_load((const i32x4*)(col + k));
+      Vec4<VT>::get(val, k, v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float s = v[u][0] * x[c[u].x] + v[u][1] * x[c[u].y] + v[u][2] * x[c[u].z] +
+                      v[u][3] * x[c[u].w];
+      acc += ok[u] ? s : 0.0f;
+    }
+  }
+  const float hs = ha * x[hc], ts = ta * x[tc];
+  acc += hv ? hs : 0.0f;
+  acc += tv ? ts : 0.0f;
+  return acc;
 }
 
 }  // namespace epet

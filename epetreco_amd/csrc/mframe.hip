@@ -29,6 +29,7 @@
 namespace epet {
 
 constexpr int MF_NXCD = 8;
+constexpr int MF_GATHERS = 16;   // gather registers per lane per round
 
 // rows -> (first row group, stride, end) for this wavefront
 __device__ __forceinline__ void mf_groups(long long ngroups, long long& g, long long& end,
@@ -43,28 +44,66 @@ __device__ __forceinline__ void mf_groups(long long ngroups, long long& g, long 
   stride = (long long)nb * (256 / WAVE);
 }
 
-// acc[t][lane] += sum over the row's entries [b, e) of v * G[col * F + t*P + f]
+// acc[t] += sum over the row's entries [b, e) of v * G[col * F + t*P + f].
+// A batch of L entries is loaded one (column, value) pair per lane, coalesced,
+// one batch ahead of its use.  Each round then broadcasts QF steps of S
+// entries inside the row's lane group and issues all their gathers
+// (MF_GATHERS registers) before accumulating, so a wavefront keeps QF * nt gathers in
+// flight.  Slots past the row end carry value 0 and a clamped column; frames
+// past F read frame F-1 and are never stored.
 template <typename VT, int L, int P, int MT>
 __device__ __forceinline__ void mf_row(long long b, long long e, int l, int F, int nt,
                                        const int* __restrict__ col,
                                        const typename VT::T* __restrict__ val,
                                        const float* __restrict__ G, float acc[MT]) {
-  constexpr int S = L / P;
+  constexpr int S = L / P;                       // entries per step
+  constexpr int NS = P;                          // steps per batch
+  constexpr int QF0 = MF_GATHERS / MT < 1 ? 1 : MF_GATHERS / MT;
+  constexpr int QF = QF0 > NS ? NS : QF0;        // steps per round
+  static_assert(NS % QF == 0, "round size");
   const int s = l / P, f = l % P;
-  for (long long base = b; base < e; base += L) {
-    const long long k = base + l;
-    const int ci = k < e ? __builtin_nontemporal_load(col + k) : 0;
-    const float cv = k < e ? VT::get(val, k) : 0.0f;
-    const int nn = (int)min((long long)L, e - base);
-    for (int q = 0; q < nn; q += S) {
-      const int c = __shfl(ci, q + s, L);
-      const float v = __shfl(cv, q + s, L);
-      const float* g = G + (long long)c * F + f;
+  int fo[MT];
 #pragma unroll
-      for (int t = 0; t < MT; t++)
-        if (t < nt && t * P + f < F) acc[t] += v * g[t * P];
+  for (int t = 0; t < MT; t++) fo[t] = min(t * P + f, F - 1);
+  if (b >= e) return;
+  long long k = b + l;
+  int ci = __builtin_nontemporal_load(col + (k < e ? k : e - 1));
+  float cv = VT::get(val, k < e ? k : e - 1);
+  cv = k < e ? cv : 0.0f;
+  for (long long base = b; base < e; base += L) {
+    const long long kn = base + L + l;
+    const long long knc = kn < e ? kn : e - 1;
+    const int cin = __builtin_nontemporal_load(col + knc);
+    float cvn = VT::get(val, knc);
+    cvn = kn < e ? cvn : 0.0f;
+#pragma unroll
+    for (int q0 = 0; q0 < NS; q0 += QF) {
+      float vq[QF];
+      float g[QF][MT];
+#pragma unroll
+      for (int qq = 0; qq < QF; qq++) {
+        const int src = (q0 + qq) * S + s;
+        const int c = __shfl(ci, src, L);
+        vq[qq] = __shfl(cv, src, L);
+        const float* gp = G + (long long)c * F;
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+          if (t < nt) g[qq][t] = gp[fo[t]];
+      }
+#pragma unroll
+      for (int qq = 0; qq < QF; qq++)
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+          if (t < nt) acc[t] += vq[qq] * g[qq][t];
     }
+    ci = cin;
+    cv = cvn;
   }
+}
+
+// lane-group sum over the S slots of each frame
+template <int L, int P, int MT>
+__device__ __forceinline__ void mf_reduce(float acc[MT]) {
 #pragma unroll
   for (int t = 0; t < MT; t++)
 #pragma unroll
@@ -72,7 +111,8 @@ __device__ __forceinline__ void mf_row(long long b, long long e, int l, int F, i
 }
 
 // Forward: E = Y / (scale * A X) (0 where the projection is 0), times the row
-// scale when fold = 1; FP64 log-likelihood per frame.
+// scale when fold = 1; FP64 log-likelihood per frame, summed per workgroup in
+// LDS and added to the global sum once per workgroup and frame.
 template <typename VT, int R, int P>
 __global__ __launch_bounds__(256) void k_mf_fwd(int m, int F, const int64_t* __restrict__ rowptr,
                                                 const int* __restrict__ col,
@@ -95,9 +135,6 @@ __global__ __launch_bounds__(256) void k_mf_fwd(int m, int F, const int64_t* __r
   const long long ngroups = ((long long)m + R - 1) / R;
   long long grp, end, stride;
   mf_groups(ngroups, grp, end, stride);
-  double ll[MT];
-#pragma unroll
-  for (int t = 0; t < MT; t++) ll[t] = 0.0;
   for (; grp < end; grp += stride) {
     const long long r = grp * R + gi;
     const bool ok = r < m;
@@ -106,6 +143,7 @@ __global__ __launch_bounds__(256) void k_mf_fwd(int m, int F, const int64_t* __r
 #pragma unroll
     for (int t = 0; t < MT; t++) acc[t] = 0.0f;
     mf_row<VT, L, P, MT>(b, e, l, F, nt, col, val, X, acc);
+    mf_reduce<L, P, MT>(acc);
     if (ok && s == 0) {
       const float sc = rowscale ? rowscale[r] : 1.0f;
 #pragma unroll
@@ -116,15 +154,10 @@ __global__ __launch_bounds__(256) void k_mf_fwd(int m, int F, const int64_t* __r
           const float yi = Y[r * F + fr];
           const float ev = yh > 0.0f ? yi / yh : 0.0f;
           E[r * F + fr] = fold ? ev * sc : ev;
-          if (yh > 0.0f) ll[t] += (double)yi * log((double)yh) - (double)yh;
+          if (yh > 0.0f) atomicAdd(&s_ll[fr], (double)yi * log((double)yh) - (double)yh);
         }
       }
     }
-  }
-  if (s == 0) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-      if (t < nt && t * P + f < F && ll[t] != 0.0) atomicAdd(&s_ll[t * P + f], ll[t]);
   }
   __syncthreads();
   if (loglik && threadIdx.x < F) unsafeAtomicAdd(&loglik[(long long)(*itp) * F + threadIdx.x],
@@ -155,6 +188,7 @@ __global__ __launch_bounds__(256) void k_mf_bwd(int n, int F, const int64_t* __r
 #pragma unroll
     for (int t = 0; t < MT; t++) acc[t] = 0.0f;
     mf_row<VT, L, P, MT>(b, e, l, F, nt, trow, tval, E, acc);
+    mf_reduce<L, P, MT>(acc);
     if (ok && s == 0) {
 #pragma unroll
       for (int t = 0; t < MT; t++) {

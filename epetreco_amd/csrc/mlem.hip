@@ -29,6 +29,7 @@ int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha,
                 float beta, float* Y);
 int pick_lanes(long long nnz, int m);
 int bp_prepare(epet_ctx* ctx, epet_matrix* A);
+bool bp_use_transpose(const epet_matrix* A);
 int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
 int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
 
@@ -95,7 +96,9 @@ struct MlemState {
 
 
 // Single frame, short rows: L lanes per row, grid-stride over rows; one FP64
-// log-likelihood atomic per workgroup.
+// log-likelihood atomic per workgroup.  fold = 1 multiplies the ratio by the row
+// scale, for a back projection through the transposed copy (which stores the
+// unscaled values).
 template <typename VT, int L>
 __global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restrict__ rowptr,
                                                    const int* __restrict__ col,
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restr
                                                    const float* __restrict__ rowscale,
                                                    const float* __restrict__ x,
                                                    const float* __restrict__ y,
-                                                   float* __restrict__ e,
+                                                   float* __restrict__ e, int fold,
                                                    const int* __restrict__ itp,
                                                    double* __restrict__ loglik) {
   __shared__ double s_w[256 / WAVE];
@@ -123,9 +126,11 @@ __global__ __launch_bounds__(256) void k_fwd_ratio(int m, const int64_t* __restr
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
     if (sub == 0) {
-      const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
+      const float sc = rowscale ? rowscale[r] : 1.0f;
+      const float yh = acc * sc;
       const float yi = y[r];
-      e[r] = yh > 0.0f ? yi / yh : 0.0f;
+      const float ev = yh > 0.0f ? yi / yh : 0.0f;
+      e[r] = fold ? ev * sc : ev;
       if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
     }
   }
@@ -141,20 +146,27 @@ __global__ __launch_bounds__(256) void k_fwd_ratio_vec(int m, const int64_t* __r
                                                        const float* __restrict__ rowscale,
                                                        const float* __restrict__ x,
                                                        const float* __restrict__ y,
-                                                       float* __restrict__ e,
+                                                       float* __restrict__ e, int fold,
                                                        const int* __restrict__ itp,
                                                        double* __restrict__ loglik) {
   __shared__ double s_w[256 / WAVE];
   const int lane = lane_id();
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
   const long long nw = (long long)gridDim.x * (256 / WAVE);
   double ll = 0.0;
-  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
-    const long long b = rowptr[r], end = rowptr[r + 1];
-    const float acc = wave_sum(row_dot_wave<VT>(b, end, lane, col, val, x));
+  long long r = (long long)blockIdx.x * (256 / WAVE) + wv;
+  long long pb = r < m ? row_bounds_load(rowptr, r, lane) : 0;
+  for (; r < m; r += nw) {
+    const long long b = lane_value_i64(pb, 0), end = lane_value_i64(pb, 1);
+    const long long rn = r + nw;
+    if (rn < m) pb = row_bounds_load(rowptr, rn, lane);
+    const float acc = b < end ? wave_sum(row_dot_wave_u<VT, 4>(b, end, lane, col, val, x)) : 0.0f;
     if (lane == 0) {
-      const float yh = acc * (rowscale ? rowscale[r] : 1.0f);
+      const float sc = rowscale ? rowscale[r] : 1.0f;
+      const float yh = acc * sc;
       const float yi = y[r];
-      e[r] = yh > 0.0f ? yi / yh : 0.0f;
+      const float ev = yh > 0.0f ? yi / yh : 0.0f;
+      e[r] = fold ? ev * sc : ev;
       if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
     }
   }
@@ -263,7 +275,7 @@ __global__ __launch_bounds__(256) void k_scale_by_sum(float* __restrict__ dst,
 
 template <typename VT>
 static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F, const float* x,
-                            const float* y, float* e, const int* itp, double* loglik) {
+                            const float* y, float* e, int fold, const int* itp, double* loglik) {
   using T = typename VT::T;
   const T* v = (const T*)A->val;
   const int m = A->m;
@@ -273,13 +285,13 @@ static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F
     const long long cap = 16ll * ncu;
     if (L == 64 && A->lanes_fwd == 0) {
       k_fwd_ratio_vec<VT><<<(unsigned)std::min<long long>((m + 3) / 4, cap), 256, 0, st>>>(
-          m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik);
+          m, A->rowptr, A->col, v, A->rowscale, x, y, e, fold, itp, loglik);
       EPET_HIP(hipGetLastError());
       return EPET_OK;
     }
 #define EPET_FR(LL)                                                                         \
   k_fwd_ratio<VT, LL><<<(unsigned)std::min<long long>(((long long)m + 256 / LL - 1) / (256 / LL), cap), \
-                        256, 0, st>>>(m, A->rowptr, A->col, v, A->rowscale, x, y, e, itp, loglik)
+                        256, 0, st>>>(m, A->rowptr, A->col, v, A->rowscale, x, y, e, fold, itp, loglik)
     switch (L) {
       case 64: EPET_FR(64); break;
       case 32: EPET_FR(32); break;
@@ -315,13 +327,18 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
     const MfArgs a{F, x, y, e, c, itp, loglik};
     rc = mf_iteration(ctx, A, a);
   } else {
+    // single frame through the transposed copy: the ratio carries the row scale
+    // and the back projection is a row-streaming product over the transpose
+    const bool via_tr = F == 1 && bp_use_transpose(A);
+    const int fold = via_tr && A->rowscale ? 1 : 0;
     switch (A->value_type) {
-      case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
-      case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
-      default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, itp, loglik); break;
+      case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
+      case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
+      default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
     }
     if (rc != EPET_OK) return rc;
-    if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
+    if (via_tr) rc = matrix_spmv(ctx, A->tr, EPET_OP_N, 1.0f, e, 0.0f, c);
+    else if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
     else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
   }
   if (rc != EPET_OK) return rc;
@@ -437,10 +454,10 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   EPET_MLEM_HIP(hipStreamSynchronize(st));
 
   // back-projection plan (allocates) and multi-frame tuning before any capture
-  if (F == 1) {
+  if (F == 1 && !bp_use_transpose(A)) {
     rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }
-  } else if (A->tr && A->mf_F != F) {
+  } else if (F > 1 && A->tr && A->mf_F != F) {
     const MfArgs a{F, x, y, e, c, itp, nullptr};
     rc = mf_tune(ctx, const_cast<epet_matrix*>(A), a, 2);
     if (rc != EPET_OK) { cleanup(); return rc; }

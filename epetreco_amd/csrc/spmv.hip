@@ -132,7 +132,9 @@ __global__ __launch_bounds__(256) void k_spmm_t(int m, int F, const int64_t* __r
 }
 
 // Forward projection, one wavefront per row, persistent grid-stride over rows.
-template <typename VT>
+// Row indices are wave-uniform; the next row's bounds are requested before the
+// current row is summed.
+template <typename VT, int U>
 __global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __restrict__ rowptr,
                                                     const int* __restrict__ col,
                                                     const typename VT::T* __restrict__ val,
@@ -140,10 +142,18 @@ __global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __rest
                                                     const float* __restrict__ x, float alpha,
                                                     float beta, float* __restrict__ y) {
   const int lane = lane_id();
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
   const long long nw = (long long)gridDim.x * (256 / WAVE);
-  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
-    const long long b = rowptr[r], e = rowptr[r + 1];
-    float acc = wave_sum(row_dot_wave<VT>(b, e, lane, col, val, x));
+  long long r = (long long)blockIdx.x * (256 / WAVE) + wv;
+  long long pb = r < m ? row_bounds_load(rowptr, r, lane) : 0;
+  for (; r < m; r += nw) {
+    const long long b = lane_value_i64(pb, 0), e = lane_value_i64(pb, 1);
+    const long long rn = r + nw;
+    if (rn < m) pb = row_bounds_load(rowptr, rn, lane);
+    float acc = 0.0f;
+    if (b < e)
+      acc = wave_sum(U == 2 ? row_dot_wave<VT>(b, e, lane, col, val, x)
+                            : row_dot_wave_u<VT, U>(b, e, lane, col, val, x));
     if (lane == 0) {
       const float v = alpha * (rowscale ? rowscale[r] : 1.0f) * acc;
       y[r] = (beta == 0.0f) ? v : v + beta * y[r];
@@ -295,10 +305,19 @@ template <typename VT>
 static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha,
                           float beta, float* y) {
   using T = typename VT::T;
+  // measurement knobs: EPET_FWD_WG workgroups per CU (default 16), EPET_FWD_U
+  // quads per lane in flight (2 or 4, default 4)
+  const char* wg = getenv("EPET_FWD_WG");
+  const long long per_cu = wg ? std::max(1, atoi(wg)) : 16;
   const long long want = ((long long)A->m + 3) / 4;
-  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, 16ll * ctx->num_cus));
-  k_spmv_n_vec<VT><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
-                                              A->rowscale, x, alpha, beta, y);
+  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, per_cu * ctx->num_cus));
+  const char* u = getenv("EPET_FWD_U");
+  if (u && u[0] == '2')
+    k_spmv_n_vec<VT, 2><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                                    A->rowscale, x, alpha, beta, y);
+  else
+    k_spmv_n_vec<VT, 4><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                                    A->rowscale, x, alpha, beta, y);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
@@ -357,17 +376,28 @@ static int launch_spmv(hipStream_t st, int op, int m, int n, long long nnz, cons
 // Kernel choice: rows of >= 64 entries on average use the vectorised wavefront
 // forward; back projection uses the LDS-tiled kernel once its plan exists
 // (EPET_BP=atomic forces the atomic kernel, for comparison).
-static bool bp_atomic_forced() {
+// EPET_BP selects the back projection for measurements: "atomic" (global float
+// atomics), "tiled" (LDS image tiles); unset: the attached transpose when there
+// is one and the values carry no row scale, else the LDS tiles.
+static bool bp_mode(const char* want) {
   const char* e = getenv("EPET_BP");
-  return e && std::string(e) == "atomic";
+  return e && std::string(e) == want;
+}
+static bool bp_atomic_forced() { return bp_mode("atomic"); }
+
+bool bp_use_transpose(const epet_matrix* A) {
+  return A->tr && !bp_mode("atomic") && !bp_mode("tiled");
 }
 
 template <typename VT>
 static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
                          float beta, float* y) {
   const double avg = A->m ? (double)A->nnz / A->m : 0.0;
-  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0)
+  const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
+  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
     return launch_fwd_vec<VT>(ctx, A, x, alpha, beta, y);
+  if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A))
+    return spmv_dispatch<VT>(ctx, A->tr, EPET_OP_N, alpha, x, beta, y);
   if (op == EPET_OP_T && !bp_atomic_forced()) {
     EPET_TRY(bp_prepare(ctx, const_cast<epet_matrix*>(A)));
     if (A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);

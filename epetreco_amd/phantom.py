@@ -60,9 +60,10 @@ def phantom(grid: VoxelGrid, name: str, seed: int = 0) -> np.ndarray:
     return voxelise(grid, ells, background=0.05, cyl_radius=0.024)
 
 
-def dynamic_series(grid: VoxelGrid, name: str, frames: int, seed: int = 0):
+def dynamic_series(grid: VoxelGrid, name: str, frames: int, seed: int = 0,
+                   counts: tuple = (1e5, 1e7)):
     """Frames t = 0..F-1 with per-ellipsoid activity a * exp(-t / tau), tau ~ U(5, 50) s,
-    and total counts log-uniform in [1e5, 1e7].  Returns (images (F, V), totals (F,))."""
+    and total counts log-uniform in `counts`.  Returns (images (F, V), totals (F,))."""
     ells = ellipsoids(name, seed)
     rng = np.random.default_rng(seed + 1)
     tau = rng.uniform(5.0, 50.0, len(ells))
@@ -70,7 +71,7 @@ def dynamic_series(grid: VoxelGrid, name: str, frames: int, seed: int = 0):
     for t in range(frames):
         acts = [e.activity * np.exp(-t / tau[k]) for k, e in enumerate(ells)]
         imgs.append(voxelise(grid, ells, activities=acts))
-    totals = np.exp(rng.uniform(np.log(1e5), np.log(1e7), frames))
+    totals = np.exp(rng.uniform(np.log(counts[0]), np.log(counts[1]), frames))
     return np.stack(imgs), totals
 
 

@@ -596,6 +596,84 @@ void eo_mlem_f64(int m, int n, const float* val, const int64_t* rowptr,
   free(s); free(c); free(e);
 }
 
+void eo_mlem_f64_frames(int m, int n, int F, const float* val, const int64_t* rowptr,
+                        const int* col, const float* Y, const float* s_in, double* X,
+                        const int* niter_f, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  nthreads = 1;
+#endif
+  const int64_t nnz = rowptr[m];
+  /* transpose by counting sort; rows of the transpose list channels ascending */
+  int64_t* tptr = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  int* trow = (int*)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  float* tval = (float*)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+  for (int64_t k = 0; k < nnz; k++) tptr[col[k] + 1]++;
+  for (int j = 0; j < n; j++) tptr[j + 1] += tptr[j];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int j = 0; j < n; j++) fill[j] = tptr[j];
+  for (int i = 0; i < m; i++)
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      const int64_t d = fill[col[k]]++;
+      trow[d] = i;
+      tval[d] = val[k];
+    }
+  free(fill);
+  double* s = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double ss = 0.0;
+  for (int j = 0; j < n; j++) ss += s_in[j];
+  for (int j = 0; j < n; j++) s[j] = s_in[j] / ss;
+  double* E = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1) * (size_t)F);
+  double* C = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1) * (size_t)F);
+  int maxit = 0;
+  for (int f = 0; f < F; f++) maxit = niter_f[f] > maxit ? niter_f[f] : maxit;
+  for (int it = 0; it < maxit; it++) {
+    /* frames in blocks of 64, entries ascending within each frame's sum */
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int i = 0; i < m; i++)
+      for (int f0 = 0; f0 < F; f0 += 64) {
+        const int nf = F - f0 < 64 ? F - f0 : 64;
+        double acc[64];
+        for (int f = 0; f < nf; f++) acc[f] = 0.0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const double v = (double)val[k];
+          const double* xr = X + (size_t)col[k] * F + f0;
+          for (int f = 0; f < nf; f++) acc[f] += v * xr[f];
+        }
+        for (int f = 0; f < nf; f++) {
+          const size_t q = (size_t)i * F + f0 + f;
+          E[q] = acc[f] > 0.0 ? (double)Y[q] / acc[f] : 0.0;
+        }
+      }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int j = 0; j < n; j++)
+      for (int f0 = 0; f0 < F; f0 += 64) {
+        const int nf = F - f0 < 64 ? F - f0 : 64;
+        double acc[64];
+        for (int f = 0; f < nf; f++) acc[f] = 0.0;
+        for (int64_t k = tptr[j]; k < tptr[j + 1]; k++) {
+          const double v = (double)tval[k];
+          const double* er = E + (size_t)trow[k] * F + f0;
+          for (int f = 0; f < nf; f++) acc[f] += v * er[f];
+        }
+        for (int f = 0; f < nf; f++) C[(size_t)j * F + f0 + f] = acc[f];
+      }
+    for (int f = 0; f < F; f++) {
+      if (it >= niter_f[f]) continue;
+      double sum = 0.0;
+      for (int j = 0; j < n; j++) {
+        const double dvs = s[j] + (s[j] == 0.0 ? 1.0 : 0.0);
+        double* xj = &X[(size_t)j * F + f];
+        *xj = *xj * C[(size_t)j * F + f] / dvs;
+        sum += *xj;
+      }
+      for (int j = 0; j < n; j++) X[(size_t)j * F + f] /= sum;
+    }
+  }
+  free(tptr); free(trow); free(tval); free(s); free(E); free(C);
+}
+
 void eo_mlem_iter_omp(int m, int n, const float* val, const int64_t* rowptr,
                       const int* col, const float* y, const float* s, float* x,
                       float* yhat, float* c, int nthreads) {

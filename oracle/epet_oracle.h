@@ -173,6 +173,14 @@ void eo_mlem_f32(int m, int n, const float* val, const int64_t* rowptr,
 void eo_mlem_f64(int m, int n, const float* val, const int64_t* rowptr,
                  const int* col, const float* y, const float* s, double* x,
                  int niter);
+/* F frames of eo_mlem_f64 sharing one matrix, frame-minor (X[j*F+f],
+ * Y[i*F+f]); frame f runs niter_f[f] iterations and is then left unchanged.
+ * The back projection walks an internal transpose whose rows keep ascending
+ * channel order, so each frame's summation order, and hence its result, is
+ * bit-identical to eo_mlem_f64 on that frame alone, for any thread count. */
+void eo_mlem_f64_frames(int m, int n, int F, const float* val, const int64_t* rowptr,
+                        const int* col, const float* Y, const float* s, double* X,
+                        const int* niter_f, int nthreads);
 /* One FP32 MLEM iteration with OpenMP (the CPU baseline).  Same result as
  * one step of eo_mlem_f32 up to summation order; yhat/e/c scratch provided. */
 void eo_mlem_iter_omp(int m, int n, const float* val, const int64_t* rowptr,

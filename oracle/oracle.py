@@ -91,6 +91,9 @@ def lib():
                                    c_float_p, c_float_p, c_float_p, ctypes.c_int]),
             "eo_mlem_f64": (None, [ctypes.c_int, ctypes.c_int, c_float_p, c_i64_p, c_int_p,
                                    c_float_p, c_float_p, c_double_p, ctypes.c_int]),
+            "eo_mlem_f64_frames": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_float_p,
+                                          c_i64_p, c_int_p, c_float_p, c_float_p, c_double_p,
+                                          c_int_p, ctypes.c_int]),
             "eo_mlem_iter_omp": (None, [ctypes.c_int, ctypes.c_int, c_float_p, c_i64_p,
                                         c_int_p, c_float_p, c_float_p, c_float_p,
                                         c_float_p, c_float_p, ctypes.c_int]),
@@ -289,6 +292,21 @@ def mlem_f64(m, n, val, rowptr, col, y, s, x0, niter):
                       ptr(np.ascontiguousarray(s, np.float32), ctypes.c_float),
                       ptr(x, ctypes.c_double), niter)
     return x
+
+
+def mlem_f64_frames(m, n, val, rowptr, col, Y, s, X0, niter, nthreads=0):
+    """F-frame FP64 MLEM (frame-minor Y (m, F), X0 (n, F)); niter is an int or
+    one count per frame."""
+    X = np.ascontiguousarray(X0, dtype=np.float64).copy()
+    F = X.shape[1]
+    its = np.broadcast_to(np.asarray(niter, np.int32), (F,)).copy()
+    lib().eo_mlem_f64_frames(m, n, F, ptr(np.ascontiguousarray(val, np.float32), ctypes.c_float),
+                             ptr(np.ascontiguousarray(rowptr, np.int64), ctypes.c_int64),
+                             ptr(np.ascontiguousarray(col, np.int32), ctypes.c_int),
+                             ptr(np.ascontiguousarray(Y, np.float32), ctypes.c_float),
+                             ptr(np.ascontiguousarray(s, np.float32), ctypes.c_float),
+                             ptr(X, ctypes.c_double), ptr(its, ctypes.c_int), nthreads)
+    return X
 
 
 def loglik(val, rowptr, col, y, x):

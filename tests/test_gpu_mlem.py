@@ -164,3 +164,32 @@ def test_multiframe_autotuner(small_problem):
     B = Matrix.from_csr(ctx, rp, col, val, prob.A.n)   # atomic back projection
     ctx.mlem(B, Y, prob.s, X2, 3)
     assert rel_l2(X1.cpu().numpy(), X2.cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_single_frame_transpose_path(small_problem, oracle, storage, monkeypatch):
+    """Single-frame MLEM and SpMV^T through the attached transpose (row scale
+    folded into the ratio for BF16) match the LDS-tiled back projection and
+    the FP64 oracle."""
+    from epetreco_amd.api import Matrix, OP_T, VAL_BF16
+    ctx, prob = small_problem
+    rp, col, val, _ = prob.A.to_torch()
+    A = Matrix.from_csr(ctx, rp, col, val, prob.A.n)
+    if storage == "bf16":
+        A = A.convert(VAL_BF16)
+    A.attach_transpose()
+    V = A.n
+    e = torch.rand(A.m, device="cuda")
+    c_tr = A.spmv(OP_T, e)
+    monkeypatch.setenv("EPET_BP", "tiled")
+    c_tiled = A.spmv(OP_T, e)
+    x_t = torch.full((V,), 1.0 / V, device="cuda")
+    ctx.mlem(A, prob.y, prob.s, x_t, 10, use_graph=True)
+    monkeypatch.delenv("EPET_BP")
+    assert rel_l2(c_tr.cpu().numpy(), c_tiled.cpu().numpy().astype(np.float64)) < 1e-6
+    x = torch.full((V,), 1.0 / V, device="cuda")
+    ctx.mlem(A, prob.y, prob.s, x, 10, use_graph=True)
+    assert rel_l2(x.cpu().numpy(), x_t.cpu().numpy().astype(np.float64)) < 1e-5
+    x64 = oracle.mlem_f64(A.m, V, val.cpu().numpy(), rp.cpu().numpy(), col.cpu().numpy(),
+                          prob.y.cpu().numpy(), prob.s.cpu().numpy(), np.full(V, 1.0 / V), 10)
+    assert rel_l2(x.cpu().numpy(), x64) < (1e-4 if storage == "f32" else 1e-3)

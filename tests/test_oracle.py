@@ -494,6 +494,27 @@ def test_mlem_omp_iteration_matches_serial(oracle):
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-5
 
 
+def test_mlem_frames_matches_single_frame_bitwise(oracle):
+    """Each frame of the multi-frame FP64 oracle equals eo_mlem_f64 on that
+    frame alone, bit for bit, including frames frozen at their own counts and
+    an empty column (zero sensitivity)."""
+    rng = np.random.default_rng(7)
+    m, n, F = 700, 90, 5
+    M, rp, col, val = _rand_csr(rng, m, n, 0.1)
+    keep = col != 17                       # column 17 stays empty
+    M[:, 17] = 0
+    rp = np.concatenate([[0], np.cumsum((M != 0).sum(1))]).astype(np.int64)
+    col, val = col[keep], val[keep]
+    Y = rng.poisson(20, (m, F)).astype(np.float32)
+    s = M.sum(0).astype(np.float32)
+    X0 = rng.random((n, F)) + 0.1
+    its = np.array([0, 1, 4, 9, 6], np.int32)
+    X = oracle.mlem_f64_frames(m, n, val, rp, col, Y, s, X0, its, nthreads=3)
+    for f in range(F):
+        xf = oracle.mlem_f64(m, n, val, rp, col, Y[:, f], s, X0[:, f], int(its[f]))
+        assert np.array_equal(X[:, f], xf), f
+
+
 # ------------------------------------------------------------------ chunk helpers
 def test_chunk_helpers(oracle):
     """wrappers.hpp:405-445 semantics: full chunks, remainder, zero past the end."""
