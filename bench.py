@@ -153,6 +153,14 @@ def main():
     m = A.m
     say(f"measured channels {m}, nonzeros {A.nnz}")
 
+    # ---- transposed copy (one-off): the back projection streams its rows
+    torch.cuda.synchronize()
+    t0 = time.time()
+    A.attach_transpose()
+    torch.cuda.synchronize()
+    tr_s = time.time() - t0
+    say(f"transposed copy built in {tr_s:.2f} s")
+
     if args.profile_kernels:
         xv = torch.rand(V, device="cuda")
         ev = torch.rand(m, device="cuda")
@@ -282,6 +290,7 @@ def main():
         "spmv_fwd_gbps": fwd_gbps,
         "spmv_bwd_gbps": bwd_gbps,
         "generation_s": gen_s,
+        "transpose_s": tr_s,
         "batched": batched,
         "roofline": {"bound": "hbm", "achieved": fwd_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": fwd_gbps / HBM_PEAK_GBPS,

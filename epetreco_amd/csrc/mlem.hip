@@ -30,6 +30,8 @@ int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha,
 int pick_lanes(long long nnz, int m);
 int bp_prepare(epet_ctx* ctx, epet_matrix* A);
 bool bp_use_transpose(const epet_matrix* A);
+int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
+                        float beta, float* y);
 int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
 int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
 
@@ -337,7 +339,7 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
       default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
     }
     if (rc != EPET_OK) return rc;
-    if (via_tr) rc = matrix_spmv(ctx, A->tr, EPET_OP_N, 1.0f, e, 0.0f, c);
+    if (via_tr) rc = matrix_bp_transpose(ctx, A->tr, 1.0f, e, 0.0f, c);
     else if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
     else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
   }

@@ -133,8 +133,9 @@ __global__ __launch_bounds__(256) void k_spmm_t(int m, int F, const int64_t* __r
 
 // Forward projection, one wavefront per row, persistent grid-stride over rows.
 // Row indices are wave-uniform; the next row's bounds are requested before the
-// current row is summed.
-template <typename VT, int U>
+// current row is summed.  BWD = 1 is the same product over the transposed copy
+// (the back projection), instantiated apart so that profiles tell them apart.
+template <typename VT, int U, int BWD>
 __global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __restrict__ rowptr,
                                                     const int* __restrict__ col,
                                                     const typename VT::T* __restrict__ val,
@@ -301,7 +302,7 @@ static int launch_bp_tiled(hipStream_t st, const epet_matrix* A, const float* x,
   return EPET_OK;
 }
 
-template <typename VT>
+template <typename VT, int BWD>
 static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha,
                           float beta, float* y) {
   using T = typename VT::T;
@@ -313,11 +314,11 @@ static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, f
   const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, per_cu * ctx->num_cus));
   const char* u = getenv("EPET_FWD_U");
   if (u && u[0] == '2')
-    k_spmv_n_vec<VT, 2><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
-                                                    A->rowscale, x, alpha, beta, y);
+    k_spmv_n_vec<VT, 2, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                                         A->rowscale, x, alpha, beta, y);
   else
-    k_spmv_n_vec<VT, 4><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
-                                                    A->rowscale, x, alpha, beta, y);
+    k_spmv_n_vec<VT, 4, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                                         A->rowscale, x, alpha, beta, y);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
@@ -389,15 +390,25 @@ bool bp_use_transpose(const epet_matrix* A) {
   return A->tr && !bp_mode("atomic") && !bp_mode("tiled");
 }
 
+// y = alpha At x + beta y over the rows of a transposed copy At (unscaled values)
+template <typename VT>
+static int bp_rows(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x, float beta,
+                   float* y) {
+  const double avg = At->m ? (double)At->nnz / At->m : 0.0;
+  if (avg >= 64.0) return launch_fwd_vec<VT, 1>(ctx, At, x, alpha, beta, y);
+  return launch_spmv<VT, int64_t>(ctx->stream, EPET_OP_N, At->m, At->n, At->nnz, At->rowptr,
+                                  At->col, At->val, nullptr, x, alpha, beta, y, 0);
+}
+
 template <typename VT>
 static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
                          float beta, float* y) {
   const double avg = A->m ? (double)A->nnz / A->m : 0.0;
   const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
   if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
-    return launch_fwd_vec<VT>(ctx, A, x, alpha, beta, y);
+    return launch_fwd_vec<VT, 0>(ctx, A, x, alpha, beta, y);
   if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A))
-    return spmv_dispatch<VT>(ctx, A->tr, EPET_OP_N, alpha, x, beta, y);
+    return bp_rows<VT>(ctx, A->tr, alpha, x, beta, y);
   if (op == EPET_OP_T && !bp_atomic_forced()) {
     EPET_TRY(bp_prepare(ctx, const_cast<epet_matrix*>(A)));
     if (A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);
@@ -412,6 +423,16 @@ int matrix_spmv(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const 
     case EPET_VAL_F32: return spmv_dispatch<ValF32>(ctx, A, op, alpha, x, beta, y);
     case EPET_VAL_BF16: return spmv_dispatch<ValBF16>(ctx, A, op, alpha, x, beta, y);
     case EPET_VAL_F16: return spmv_dispatch<ValF16>(ctx, A, op, alpha, x, beta, y);
+  }
+  EPET_FAIL(EPET_ERR_STATE, "unknown value type");
+}
+
+int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
+                        float beta, float* y) {
+  switch (At->value_type) {
+    case EPET_VAL_F32: return bp_rows<ValF32>(ctx, At, alpha, x, beta, y);
+    case EPET_VAL_BF16: return bp_rows<ValBF16>(ctx, At, alpha, x, beta, y);
+    case EPET_VAL_F16: return bp_rows<ValF16>(ctx, At, alpha, x, beta, y);
   }
   EPET_FAIL(EPET_ERR_STATE, "unknown value type");
 }
