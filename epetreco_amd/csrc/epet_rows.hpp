@@ -158,4 +158,37 @@ __device__ __forceinline__ float row_dot_wave_u(long long b, long long e, int la
   return acc;
 }
 
+// Same sum with the lane-strided layout: lane l takes entries base + l + 64 j,
+// j < J, so each gather instruction reads x at 64 consecutive entries of the
+// row (neighbouring voxels, few distinct cache lines); all 2J loads of a pass
+// are issued before the first gather.  Slots past the row end load the last
+// entry and are zeroed by a select.  Requires b < e.
+template <typename VT, int J>
+__device__ __forceinline__ float row_dot_wave_s(long long b, long long e, int lane,
+                                                const int* __restrict__ col,
+                                                const typename VT::T* __restrict__ val,
+                                                const float* __restrict__ x) {
+  float acc = 0.0f;
+  const long long last = e - 1;
+  for (long long base = b; base < e; base += (long long)J * WAVE) {
+    int c[J];
+    float v[J];
+    bool ok[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const long long k = base + lane + (long long)j * WAVE;
+      ok[j] = k < e;
+      const long long kc = ok[j] ? k : last;
+      c[j] = ldcol(col, kc);
+      v[j] = VT::get(val, kc);
+    }
+    float g[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) g[j] = x[c[j]];
+#pragma unroll
+    for (int j = 0; j < J; j++) acc += ok[j] ? v[j] * g[j] : 0.0f;
+  }
+  return acc;
+}
+
 }  // namespace epet

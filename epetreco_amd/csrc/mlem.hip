@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_fwd_ratio_vec(int m, const int64_t* __r
     const long long b = lane_value_i64(pb, 0), end = lane_value_i64(pb, 1);
     const long long rn = r + nw;
     if (rn < m) pb = row_bounds_load(rowptr, rn, lane);
-    const float acc = b < end ? wave_sum(row_dot_wave_u<VT, 4>(b, end, lane, col, val, x)) : 0.0f;
+    const float acc = b < end ? wave_sum(row_dot_wave_s<VT, 16>(b, end, lane, col, val, x)) : 0.0f;
     if (lane == 0) {
       const float sc = rowscale ? rowscale[r] : 1.0f;
       const float yh = acc * sc;

@@ -153,8 +153,9 @@ __global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __rest
     if (rn < m) pb = row_bounds_load(rowptr, rn, lane);
     float acc = 0.0f;
     if (b < e)
-      acc = wave_sum(U == 2 ? row_dot_wave<VT>(b, e, lane, col, val, x)
-                            : row_dot_wave_u<VT, U>(b, e, lane, col, val, x));
+      acc = wave_sum(U == 2   ? row_dot_wave<VT>(b, e, lane, col, val, x)
+                     : U == 4 ? row_dot_wave_u<VT, 4>(b, e, lane, col, val, x)
+                              : row_dot_wave_s<VT, U>(b, e, lane, col, val, x));
     if (lane == 0) {
       const float v = alpha * (rowscale ? rowscale[r] : 1.0f) * acc;
       y[r] = (beta == 0.0f) ? v : v + beta * y[r];
@@ -306,19 +307,27 @@ template <typename VT, int BWD>
 static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha,
                           float beta, float* y) {
   using T = typename VT::T;
-  // measurement knobs: EPET_FWD_WG workgroups per CU (default 16), EPET_FWD_U
-  // quads per lane in flight (2 or 4, default 4)
+  // measurement knob: EPET_FWD_WG workgroups per CU (default 16)
   const char* wg = getenv("EPET_FWD_WG");
   const long long per_cu = wg ? std::max(1, atoi(wg)) : 16;
   const long long want = ((long long)A->m + 3) / 4;
   const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, per_cu * ctx->num_cus));
+  // EPET_FWD_U: "2"/"4" quads per lane (vector layout), "8" lane-strided with
+  // 8 entries per lane; default lane-strided with 16 entries per lane
   const char* u = getenv("EPET_FWD_U");
+  const T* v = (const T*)A->val;
   if (u && u[0] == '2')
-    k_spmv_n_vec<VT, 2, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
-                                                         A->rowscale, x, alpha, beta, y);
+    k_spmv_n_vec<VT, 2, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
+                                                         x, alpha, beta, y);
+  else if (u && u[0] == '4')
+    k_spmv_n_vec<VT, 4, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
+                                                         x, alpha, beta, y);
+  else if (u && u[0] == '8')
+    k_spmv_n_vec<VT, 8, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
+                                                         x, alpha, beta, y);
   else
-    k_spmv_n_vec<VT, 4, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
-                                                         A->rowscale, x, alpha, beta, y);
+    k_spmv_n_vec<VT, 16, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
+                                                          x, alpha, beta, y);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }

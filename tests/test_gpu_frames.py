@@ -116,20 +116,6 @@ def oracle_mlem(*a):
     return O.mlem_f64_frames(*a)
 
 
-def _decoded(M):
-    """Stored operator as FP32 values: decoded BF16/FP16 times the row scale."""
-    rp, col, val, sc = M.to_torch()
-    rp_h = rp.cpu().numpy()
-    raw = val.cpu().numpy().view(np.uint16)
-    from epetreco_amd.api import VAL_BF16
-    if M.value_type == VAL_BF16:
-        v = (raw.astype(np.uint32) << 16).view(np.float32)
-    else:                      # FP16
-        v = raw.view(np.float16).astype(np.float32)
-    rows = np.repeat(np.arange(M.m), np.diff(rp_h))
-    return rp_h, col.cpu().numpy(), (v * sc.cpu().numpy()[rows]).astype(np.float32)
-
-
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 def test_reduced_precision_storage_104(large_problem, storage):
     from epetreco_amd.api import VAL_BF16, VAL_F16
@@ -146,7 +132,8 @@ def test_reduced_precision_storage_104(large_problem, storage):
     if storage == "f32":
         assert e_gpu.max() < 1e-4
         return
-    rp, col, vq = _decoded(M)
+    from helpers import decoded
+    rp, col, vq = decoded(M)
     xq = oracle_mlem(A.m, V, vq, rp, col, Y.cpu().numpy(), sens.cpu().numpy(),
                      np.full((V, F), 1.0 / V), 30)
     e_kernel = np.array([rel_l2(Xh[:, f], xq[:, f]) for f in range(F)])
