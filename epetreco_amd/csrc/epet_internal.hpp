@@ -94,15 +94,23 @@ struct epet_matrix {
   bool bp_failed = false;
   bool cols_strict = true;   // columns strictly ascending within every row
   // transposed copy (CSR of A^T: rows = voxels, entries (row index, value)),
-  // used by the multi-frame back projection; per-row scales of A are folded
-  // into the back-projected vector instead
+  // used by the back projections; per-row scales of A are folded into the
+  // back-projected vector instead
   epet_matrix* tr = nullptr;
+  // on a transposed copy: entry offsets (relative to the row start) of the
+  // XB_BLOCKS contiguous blocks of its columns, (m x (XB_BLOCKS+1)), and the
+  // per-block partial products (XB_BLOCKS x m)
+  uint32_t* xb_toff = nullptr;
+  float* xb_partial = nullptr;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured
   int mf_F = 0, mf_fwd_R = 1, mf_fwd_P = 32, mf_bwd_R = 1, mf_bwd_P = 32;
   float mf_fwd_ms = 0.0f, mf_bwd_ms = 0.0f;
   ~epet_matrix();
 };
+
+// column blocks of the blocked back projection: one per XCD of the MI355X
+constexpr int XB_BLOCKS = 8;
 
 namespace epet {
 

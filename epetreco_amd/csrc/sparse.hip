@@ -350,10 +350,13 @@ __global__ __launch_bounds__(256) void k_tr_rowptr(int n, int B, long long nnz,
 epet_matrix::~epet_matrix() {
   if (bp_toff) (void)hipFree(bp_toff);
   if (bp_partial) (void)hipFree(bp_partial);
+  if (xb_toff) (void)hipFree(xb_toff);
+  if (xb_partial) (void)hipFree(xb_partial);
   if (tr) epet_matrix_destroy(tr);
 }
 
 namespace epet {
+void xb_prepare(epet_ctx* ctx, epet_matrix* At);
 int matrix_alloc(epet_matrix** out, int m, int n, long long nnz, int value_type) {
   epet_matrix* A = new epet_matrix();
   A->m = m;
@@ -658,6 +661,7 @@ int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A) {
   (void)hipFreeAsync(off, st);
   EPET_HIP(hipStreamSynchronize(st));
   T->bp_failed = true;
+  xb_prepare(ctx, T);   // optional: without a plan the plain row kernel runs
   A->tr = T;
   return EPET_OK;
 }

@@ -387,8 +387,9 @@ static int launch_spmv(hipStream_t st, int op, int m, int n, long long nnz, cons
 // forward; back projection uses the LDS-tiled kernel once its plan exists
 // (EPET_BP=atomic forces the atomic kernel, for comparison).
 // EPET_BP selects the back projection for measurements: "atomic" (global float
-// atomics), "tiled" (LDS image tiles); unset: the attached transpose when there
-// is one and the values carry no row scale, else the LDS tiles.
+// atomics), "tiled" (LDS image tiles), "rows" (plain row kernel over the
+// transposed copy); unset: the blocked kernel over the attached transpose when
+// there is one and the values carry no row scale, else the LDS tiles.
 static bool bp_mode(const char* want) {
   const char* e = getenv("EPET_BP");
   return e && std::string(e) == want;
@@ -399,11 +400,81 @@ bool bp_use_transpose(const epet_matrix* A) {
   return A->tr && !bp_mode("atomic") && !bp_mode("tiled");
 }
 
+// Blocked back projection over a transposed copy: the columns of At (channel
+// rows of A) are cut into XB_BLOCKS contiguous blocks and workgroup b sums the
+// entries of block b mod XB_BLOCKS of its rows.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so each XCD gathers only its block of x (the
+// ratio vector), which stays resident in its L2; the placement only affects
+// speed.  Partial products are summed in block order by k_bp_xb_reduce.
+template <typename VT, int J>
+__global__ __launch_bounds__(256) void k_bp_xb(int n, const int64_t* __restrict__ tptr,
+                                               const uint32_t* __restrict__ toff,
+                                               const int* __restrict__ tcol,
+                                               const typename VT::T* __restrict__ tval,
+                                               const float* __restrict__ x,
+                                               float* __restrict__ partial) {
+  const int lane = lane_id();
+  const int kb = blockIdx.x % XB_BLOCKS;
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
+  const long long nw = (long long)(gridDim.x / XB_BLOCKS) * (256 / WAVE);
+  for (long long j = (long long)(blockIdx.x / XB_BLOCKS) * (256 / WAVE) + wv; j < n; j += nw) {
+    const long long rb = tptr[j];
+    const uint32_t* o = toff + j * (XB_BLOCKS + 1) + kb;
+    const long long b = rb + o[0], e = rb + o[1];
+    float acc = 0.0f;
+    if (b < e) acc = wave_sum(row_dot_wave_s<VT, J>(b, e, lane, tcol, tval, x));
+    if (lane == 0) partial[(long long)kb * n + j] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bp_xb_reduce(int n, const float* __restrict__ partial,
+                                                      float alpha, float beta,
+                                                      float* __restrict__ y) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < XB_BLOCKS; k++) s += partial[(long long)k * n + j];
+  y[j] = beta == 0.0f ? alpha * s : alpha * s + beta * y[j];
+}
+
+// Plan of the blocked back projection for a transposed copy (entry offsets of
+// every column-block boundary in every row); left absent when it cannot be
+// allocated.
+void xb_prepare(epet_ctx* ctx, epet_matrix* At) {
+  if (At->xb_toff || At->m == 0) return;
+  const long long nt = (long long)At->m * (XB_BLOCKS + 1);
+  if (hipMalloc((void**)&At->xb_toff, sizeof(uint32_t) * nt) != hipSuccess) {
+    (void)hipGetLastError();
+    At->xb_toff = nullptr;
+    return;
+  }
+  if (hipMalloc((void**)&At->xb_partial, sizeof(float) * (size_t)At->m * XB_BLOCKS) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(At->xb_toff);
+    At->xb_toff = nullptr;
+    At->xb_partial = nullptr;
+    return;
+  }
+  const int bs = (At->n + XB_BLOCKS - 1) / XB_BLOCKS;
+  k_bp_plan<<<grid_for(nt, 256, 1ll << 31), 256, 0, ctx->stream>>>(At->m, XB_BLOCKS, bs, At->rowptr,
+                                                                   At->col, At->xb_toff);
+}
+
 // y = alpha At x + beta y over the rows of a transposed copy At (unscaled values)
 template <typename VT>
 static int bp_rows(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x, float beta,
                    float* y) {
   const double avg = At->m ? (double)At->nnz / At->m : 0.0;
+  if (At->xb_toff && avg >= 512.0 && !bp_mode("rows")) {
+    const unsigned g = (unsigned)(XB_BLOCKS * std::max(1, 2 * ctx->num_cus));
+    k_bp_xb<VT, 16><<<g, 256, 0, ctx->stream>>>(At->m, At->rowptr, At->xb_toff, At->col,
+                                                 (const typename VT::T*)At->val, x, At->xb_partial);
+    k_bp_xb_reduce<<<grid_for(At->m, 256, 1ll << 31), 256, 0, ctx->stream>>>(At->m, At->xb_partial,
+                                                                            alpha, beta, y);
+    EPET_HIP(hipGetLastError());
+    return EPET_OK;
+  }
   if (avg >= 64.0) return launch_fwd_vec<VT, 1>(ctx, At, x, alpha, beta, y);
   return launch_spmv<VT, int64_t>(ctx->stream, EPET_OP_N, At->m, At->n, At->nnz, At->rowptr,
                                   At->col, At->val, nullptr, x, alpha, beta, y, 0);

@@ -153,6 +153,34 @@ def test_spmv_multi_tile_back_projection(gpu_ctx_factory, oracle, n, avg):
         np.testing.assert_allclose(c, cr, rtol=2e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("m,n,avg", [(60000, 3000, 100), (50001, 777, 40), (9000, 1000, 70)])
+def test_blocked_transposed_back_projection(gpu_ctx_factory, oracle, monkeypatch, m, n, avg):
+    """Back projection through the transposed copy with long transposed rows
+    (>= 512 entries: the column-blocked kernel), with ragged block boundaries
+    (m not a multiple of 8) and empty rows, vs the FP64 oracle and the plain
+    row kernel, for beta 0, 1 and 0.5."""
+    from epetreco_amd.api import Matrix, OP_T
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(m + n)
+    rp, col, val = _random_csr(rng, m, n, avg)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    A.attach_transpose()
+    _say("blocked back projection", m, n, A.nnz, "avg transposed row", A.nnz / n)
+    e = rng.random(m).astype(np.float32)
+    e[::5] = 0.0
+    y0 = rng.random(n).astype(np.float32)
+    for beta in (0.0, 1.0, 0.5):
+        c = A.spmv(OP_T, _t(e, torch.float32), _t(y0, torch.float32).clone(), alpha=1.5,
+                   beta=beta).cpu().numpy()
+        cr = oracle.csrmv(1, m, n, 1.5, val, rp, col, e, beta, y0)
+        np.testing.assert_allclose(c, cr, rtol=2e-5, atol=1e-6)
+    monkeypatch.setenv("EPET_BP", "rows")
+    c_rows = A.spmv(OP_T, _t(e, torch.float32)).cpu().numpy()
+    monkeypatch.delenv("EPET_BP")
+    c_blk = A.spmv(OP_T, _t(e, torch.float32)).cpu().numpy()
+    np.testing.assert_allclose(c_blk, c_rows, rtol=2e-6, atol=1e-7)
+
+
 def test_unsorted_columns_fall_back(gpu_ctx_factory, oracle):
     """Columns not ascending within rows: still correct (atomic back projection)."""
     from epetreco_amd.api import Matrix, OP_T
