@@ -32,6 +32,7 @@ SOURCES = {
     "spmv.hip": [],
     "mlem.hip": [],
     "mframe.hip": [],
+    "lds.hip": [],
 }
 
 

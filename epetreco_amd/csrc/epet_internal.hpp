@@ -102,6 +102,13 @@ struct epet_matrix {
   // per-block partial products (XB_BLOCKS x m)
   uint32_t* xb_toff = nullptr;
   float* xb_partial = nullptr;
+  // LDS-tiled products: the gathered vector is cut into tiles of LDS_TILE
+  // entries; entry offsets of every tile boundary in every row (m x (lds_T+1))
+  // and the per-tile partial products (lds_T x m)
+  int lds_T = 0;
+  uint32_t* lds_toff = nullptr;
+  float* lds_partial = nullptr;
+  bool lds_failed = false;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured
   int mf_F = 0, mf_fwd_R = 1, mf_fwd_P = 32, mf_bwd_R = 1, mf_bwd_P = 32;
@@ -111,6 +118,9 @@ struct epet_matrix {
 
 // column blocks of the blocked back projection: one per XCD of the MI355X
 constexpr int XB_BLOCKS = 8;
+// LDS tile of the LDS-tiled row products: 32768 floats = 128 KB of the 160 KB
+// of LDS per workgroup on gfx950
+constexpr int LDS_TILE = 32768;
 
 namespace epet {
 

@@ -32,6 +32,10 @@ int bp_prepare(epet_ctx* ctx, epet_matrix* A);
 bool bp_use_transpose(const epet_matrix* A);
 int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
                         float beta, float* y);
+int lds_prepare(epet_ctx* ctx, epet_matrix* A);
+bool lds_ready(const epet_matrix* A);
+int lds_fwd_ratio(epet_ctx* ctx, const epet_matrix* A, const float* x, const float* y, float* e,
+                  int fold, const int* itp, double* loglik);
 int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
 int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
 
@@ -333,7 +337,8 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
     // and the back projection is a row-streaming product over the transpose
     const bool via_tr = F == 1 && bp_use_transpose(A);
     const int fold = via_tr && A->rowscale ? 1 : 0;
-    switch (A->value_type) {
+    if (F == 1 && lds_ready(A)) rc = lds_fwd_ratio(ctx, A, x, y, e, fold, itp, loglik);
+    else switch (A->value_type) {
       case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
       case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
       default: rc = launch_fwd_ratio<ValF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
@@ -456,8 +461,10 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   EPET_MLEM_HIP(hipStreamSynchronize(st));
 
   // back-projection plan (allocates) and multi-frame tuning before any capture
-  if (F == 1 && !bp_use_transpose(A)) {
-    rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
+  if (F == 1) {
+    rc = lds_prepare(ctx, const_cast<epet_matrix*>(A));
+    if (rc == EPET_OK && bp_use_transpose(A)) rc = lds_prepare(ctx, A->tr);
+    if (rc == EPET_OK && !bp_use_transpose(A)) rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }
   } else if (F > 1 && A->tr && A->mf_F != F) {
     const MfArgs a{F, x, y, e, c, itp, nullptr};

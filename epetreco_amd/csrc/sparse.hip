@@ -352,6 +352,8 @@ epet_matrix::~epet_matrix() {
   if (bp_partial) (void)hipFree(bp_partial);
   if (xb_toff) (void)hipFree(xb_toff);
   if (xb_partial) (void)hipFree(xb_partial);
+  if (lds_toff) (void)hipFree(lds_toff);
+  if (lds_partial) (void)hipFree(lds_partial);
   if (tr) epet_matrix_destroy(tr);
 }
 

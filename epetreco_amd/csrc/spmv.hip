@@ -438,6 +438,14 @@ __global__ __launch_bounds__(256) void k_bp_xb_reduce(int n, const float* __rest
   y[j] = beta == 0.0f ? alpha * s : alpha * s + beta * y[j];
 }
 
+// Entry offsets of every column-tile boundary (tiles of TS columns) in every
+// row: toff[r (T+1) + t] = first entry of row r with column >= t TS.
+void launch_row_plan(hipStream_t st, int m, int T, int TS, const int64_t* rowptr, const int* col,
+                     uint32_t* toff) {
+  const long long nt = (long long)m * (T + 1);
+  if (nt > 0) k_bp_plan<<<grid_for(nt, 256, 1ll << 31), 256, 0, st>>>(m, T, TS, rowptr, col, toff);
+}
+
 // Plan of the blocked back projection for a transposed copy (entry offsets of
 // every column-block boundary in every row); left absent when it cannot be
 // allocated.
@@ -461,10 +469,16 @@ void xb_prepare(epet_ctx* ctx, epet_matrix* At) {
                                                                    At->col, At->xb_toff);
 }
 
+int lds_prepare(epet_ctx* ctx, epet_matrix* A);
+bool lds_ready(const epet_matrix* A);
+int lds_spmv(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha, float beta,
+             float* y, bool use_scale, bool bwd);
+
 // y = alpha At x + beta y over the rows of a transposed copy At (unscaled values)
 template <typename VT>
 static int bp_rows(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x, float beta,
                    float* y) {
+  if (lds_ready(At)) return lds_spmv(ctx, At, x, alpha, beta, y, false, true);
   const double avg = At->m ? (double)At->nnz / At->m : 0.0;
   if (At->xb_toff && avg >= 512.0 && !bp_mode("rows")) {
     const unsigned g = (unsigned)(XB_BLOCKS * std::max(1, 2 * ctx->num_cus));
@@ -487,8 +501,14 @@ static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alph
   const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
   if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
     return launch_fwd_vec<VT, 0>(ctx, A, x, alpha, beta, y);
-  if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A))
+  if (op == EPET_OP_N) {
+    EPET_TRY(lds_prepare(ctx, const_cast<epet_matrix*>(A)));
+    if (lds_ready(A)) return lds_spmv(ctx, A, x, alpha, beta, y, true, false);
+  }
+  if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A)) {
+    EPET_TRY(lds_prepare(ctx, A->tr));
     return bp_rows<VT>(ctx, A->tr, alpha, x, beta, y);
+  }
   if (op == EPET_OP_T && !bp_atomic_forced()) {
     EPET_TRY(bp_prepare(ctx, const_cast<epet_matrix*>(A)));
     if (A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);

@@ -200,3 +200,47 @@ def test_single_frame_transpose_path(small_problem, oracle, storage, monkeypatch
     x64 = oracle.mlem_f64(A.m, V, vq, rq, cq, prob.y.cpu().numpy(), prob.s.cpu().numpy(),
                           np.full(V, 1.0 / V), 10)
     assert rel_l2(x.cpu().numpy(), x64) < REL_L2
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_lds_tiled_products_and_mlem(gpu_ctx_factory, oracle, storage, monkeypatch):
+    """LDS-tiled row products on rows long enough to use them (avg 300 entries)
+    over 70,001 columns (three 32768-entry tiles, the last ragged) and over the
+    transposed copy: forward and back projection vs the FP64 oracle on the
+    stored operator, and 8 MLEM iterations (graph) vs the same iterations with
+    the LDS path off (rel-L2 <= 1e-5) and vs the FP64 oracle (<= 1e-4)."""
+    from epetreco_amd.api import Matrix, OP_N, OP_T, VAL_BF16
+    from helpers import decoded
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(11)
+    m, n, avg = 6000, 70001, 300
+    lens = rng.poisson(avg, m)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    val = rng.random(rp[-1]).astype(np.float32)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    if storage == "bf16":
+        A = A.convert(VAL_BF16)
+    A.attach_transpose()
+    rq, cq, vq = decoded(A)
+    x = rng.random(n).astype(np.float32)
+    e = rng.random(m).astype(np.float32)
+    y = A.spmv(OP_N, torch.as_tensor(x, device="cuda")).cpu().numpy()
+    np.testing.assert_allclose(y, oracle.csrmv(0, m, n, 1.0, vq, rq, cq, x, 0.0, np.zeros(m, np.float32)),
+                               rtol=2e-5, atol=1e-6)
+    if storage == "f32":
+        c = A.spmv(OP_T, torch.as_tensor(e, device="cuda")).cpu().numpy()
+        np.testing.assert_allclose(c, oracle.csrmv(1, m, n, 1.0, vq, rq, cq, e, 0.0,
+                                                   np.zeros(n, np.float32)), rtol=2e-5, atol=1e-6)
+    counts = rng.poisson(30.0, m).astype(np.float32)
+    s = torch.as_tensor(np.bincount(cq, weights=vq, minlength=n).astype(np.float32), device="cuda")
+    yy = torch.as_tensor(counts, device="cuda")
+    x1 = torch.full((n,), 1.0 / n, device="cuda")
+    ctx.mlem(A, yy, s, x1, 8, use_graph=True)
+    monkeypatch.setenv("EPET_LDS", "0")
+    x2 = torch.full((n,), 1.0 / n, device="cuda")
+    ctx.mlem(A, yy, s, x2, 8, use_graph=True)
+    monkeypatch.delenv("EPET_LDS")
+    assert rel_l2(x1.cpu().numpy(), x2.cpu().numpy().astype(np.float64)) < 1e-5
+    x64 = oracle.mlem_f64(m, n, vq, rq, cq, counts, s.cpu().numpy(), np.full(n, 1.0 / n), 8)
+    assert rel_l2(x1.cpu().numpy(), x64) < REL_L2
