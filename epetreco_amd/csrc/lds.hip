@@ -56,16 +56,27 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
   const long long r0 = g * rows_per_wg;
   const long long r1 = std::min<long long>((long long)m, r0 + rows_per_wg);
   const unsigned last = (unsigned)(cnt - 1);
-  for (long long rb = r0 + (long long)w * R; rb < r1; rb += (long long)LDS_WAVES * R) {
-    const long long r = rb + gi;
-    const bool ok = r < r1;
-    long long s0 = 0, s1 = 0;
-    if (ok) {
+  const long long step = (long long)LDS_WAVES * R;
+  // segment [s0, s1) of row rbx + gi in tile t; empty past the range
+  auto segment = [&](long long rbx, long long& s0, long long& s1) {
+    s0 = 0;
+    s1 = 0;
+    const long long r = rbx + gi;
+    if (r < r1) {
       const long long b = rowptr[r];
       const uint32_t* o = toff + r * (T + 1) + t;
       s0 = b + o[0];
       s1 = b + o[1];
     }
+  };
+  long long rb = r0 + (long long)w * R;
+  long long n0, n1;
+  segment(rb, n0, n1);
+  for (; rb < r1; rb += step) {
+    const long long r = rb + gi;
+    const bool ok = r < r1;
+    const long long s0 = n0, s1 = n1;
+    segment(rb + step, n0, n1);   // the next segment's bounds, requested before this one's entries
     float acc = 0.0f;
     for (long long base = s0; base < s1; base += (long long)L * J) {
       int c[J];
@@ -165,7 +176,7 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
-  if (seg >= 384.0) launch_rows_lds<VT, 1, 8, BWD>(ctx, A, x);
+  if (seg >= 384.0) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
   else if (seg >= 96.0) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
   else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
 }

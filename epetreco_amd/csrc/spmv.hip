@@ -498,13 +498,13 @@ template <typename VT>
 static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const float* x,
                          float beta, float* y) {
   const double avg = A->m ? (double)A->nnz / A->m : 0.0;
-  const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
-  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
-    return launch_fwd_vec<VT, 0>(ctx, A, x, alpha, beta, y);
   if (op == EPET_OP_N) {
     EPET_TRY(lds_prepare(ctx, const_cast<epet_matrix*>(A)));
     if (lds_ready(A)) return lds_spmv(ctx, A, x, alpha, beta, y, true, false);
   }
+  const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
+  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
+    return launch_fwd_vec<VT, 0>(ctx, A, x, alpha, beta, y);
   if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A)) {
     EPET_TRY(lds_prepare(ctx, A->tr));
     return bp_rows<VT>(ctx, A->tr, alpha, x, beta, y);
