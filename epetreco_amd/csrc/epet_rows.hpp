@@ -11,22 +11,30 @@ namespace epet {
 
 // stored value -> float (reduced-precision values are multiplied by the row
 // scale by the caller)
+// get: non-temporal (streaming) load; get_cached: ordinary load
 struct ValF32 {
   using T = float;
   static __device__ __forceinline__ float get(const T* p, long long k) {
     return __builtin_nontemporal_load(p + k);
   }
+  static __device__ __forceinline__ float get_cached(const T* p, long long k) { return p[k]; }
 };
 struct ValBF16 {
   using T = uint16_t;
   static __device__ __forceinline__ float get(const T* p, long long k) {
     return __uint_as_float((uint32_t)__builtin_nontemporal_load(p + k) << 16);
   }
+  static __device__ __forceinline__ float get_cached(const T* p, long long k) {
+    return __uint_as_float((uint32_t)p[k] << 16);
+  }
 };
 struct ValF16 {
   using T = uint16_t;
   static __device__ __forceinline__ float get(const T* p, long long k) {
     return __half2float(__ushort_as_half(__builtin_nontemporal_load(p + k)));
+  }
+  static __device__ __forceinline__ float get_cached(const T* p, long long k) {
+    return __half2float(__ushort_as_half(p[k]));
   }
 };
 

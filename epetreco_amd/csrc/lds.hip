@@ -30,8 +30,9 @@ constexpr int LDS_WAVES = LDS_BLOCK / WAVE;
 
 // R rows per wavefront (L = 64/R lanes per row), J entries per lane per pass;
 // BWD = 1 marks the back projection over a transposed copy (a separate
-// instantiation, so that profiles tell the two products apart).
-template <typename VT, int R, int J, int BWD>
+// instantiation, so that profiles tell the two products apart); NT = 1 loads
+// the entry stream with the non-temporal hint.
+template <typename VT, int R, int J, int BWD, int NT>
 __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, long long rows_per_wg,
                                                         const int64_t* __restrict__ rowptr,
                                                         const uint32_t* __restrict__ toff,
@@ -87,8 +88,13 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
         const long long k = base + l + (long long)j * L;
         in[j] = k < s1;
         const long long kc = in[j] ? k : s1 - 1;
-        c[j] = __builtin_nontemporal_load(col + kc);
-        v[j] = VT::get(val, kc);
+        if (NT) {
+          c[j] = __builtin_nontemporal_load(col + kc);
+          v[j] = VT::get(val, kc);
+        } else {
+          c[j] = col[kc];
+          v[j] = VT::get_cached(val, kc);
+        }
       }
 #pragma unroll
       for (int j = 0; j < J; j++) {
@@ -148,37 +154,53 @@ __global__ __launch_bounds__(256) void k_lds_ratio(int m, int T, const float* __
   }
 }
 
-template <typename VT, int R, int J, int BWD>
+template <typename VT, int R, int J, int BWD, int NT>
 static void set_lds_attr() {
   static bool done = false;
   if (!done) {
-    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD>,
+    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD, NT>,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               LDS_TILE * (int)sizeof(float));
     done = true;
   }
 }
 
-template <typename VT, int R, int J, int BWD>
+template <typename VT, int R, int J, int BWD, int NT>
 static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
-  set_lds_attr<VT, R, J, BWD>();
+  set_lds_attr<VT, R, J, BWD, NT>();
   // each workgroup owns its CU's LDS; about four workgroups per CU in all, so
   // that tiles of unequal weight even out
   const long long G = std::max<long long>(1, (4ll * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
-  k_rows_lds<VT, R, J, BWD><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
+  k_rows_lds<VT, R, J, BWD, NT><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
                              ctx->stream>>>(
       A->m, A->n, A->lds_T, rows_per_wg, A->rowptr, A->lds_toff, A->col,
       (const typename VT::T*)A->val, x, A->lds_partial);
 }
 
+// Rows per wavefront from the mean segment length.  Entries are loaded with
+// ordinary (cached) loads: with R > 1 a row group's load covers only part of
+// a cache line, and the line's other part is read by the next load, which a
+// streaming hint would send back to HBM.  Measurement knobs: EPET_LDS_R (1, 2,
+// 4 or 16) forces R, EPET_LDS_NT=1 selects non-temporal loads.
+template <typename VT, int BWD, int NT>
+static void rows_lds_r(epet_ctx* ctx, const epet_matrix* A, const float* x, int R) {
+  if (R == 1) launch_rows_lds<VT, 1, 16, BWD, NT>(ctx, A, x);
+  else if (R == 2) launch_rows_lds<VT, 2, 16, BWD, NT>(ctx, A, x);
+  else if (R == 4) launch_rows_lds<VT, 4, 16, BWD, NT>(ctx, A, x);
+  else launch_rows_lds<VT, 16, 16, BWD, NT>(ctx, A, x);
+}
+
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
-  if (seg >= 384.0) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
-  else if (seg >= 96.0) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
-  else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
+  int R = seg >= 384.0 ? 2 : seg >= 48.0 ? 4 : 16;
+  const char* er = getenv("EPET_LDS_R");
+  if (er) R = atoi(er);
+  const char* en = getenv("EPET_LDS_NT");
+  if (en && en[0] == '1') rows_lds_r<VT, BWD, 1>(ctx, A, x, R);
+  else rows_lds_r<VT, BWD, 0>(ctx, A, x, R);
 }
 
 template <int BWD>

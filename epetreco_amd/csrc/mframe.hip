@@ -66,15 +66,16 @@ __device__ __forceinline__ void mf_row(long long b, long long e, int l, int F, i
 #pragma unroll
   for (int t = 0; t < MT; t++) fo[t] = min(t * P + f, F - 1);
   if (b >= e) return;
+  // ordinary loads: with R > 1 a row group reads part of a cache line per load
   long long k = b + l;
-  int ci = __builtin_nontemporal_load(col + (k < e ? k : e - 1));
-  float cv = VT::get(val, k < e ? k : e - 1);
+  int ci = col[k < e ? k : e - 1];
+  float cv = VT::get_cached(val, k < e ? k : e - 1);
   cv = k < e ? cv : 0.0f;
   for (long long base = b; base < e; base += L) {
     const long long kn = base + L + l;
     const long long knc = kn < e ? kn : e - 1;
-    const int cin = __builtin_nontemporal_load(col + knc);
-    float cvn = VT::get(val, knc);
+    const int cin = col[knc];
+    float cvn = VT::get_cached(val, knc);
     cvn = kn < e ? cvn : 0.0f;
 #pragma unroll
     for (int q0 = 0; q0 < NS; q0 += QF) {
