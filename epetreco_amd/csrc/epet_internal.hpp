@@ -108,6 +108,7 @@ struct epet_matrix {
   int lds_T = 0;
   uint32_t* lds_toff = nullptr;
   float* lds_partial = nullptr;
+  uint16_t* lds_col16 = nullptr;   // tile-local column of every entry (col mod LDS_TILE)
   bool lds_failed = false;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured

@@ -31,12 +31,13 @@ constexpr int LDS_WAVES = LDS_BLOCK / WAVE;
 // R rows per wavefront (L = 64/R lanes per row), J entries per lane per pass;
 // BWD = 1 marks the back projection over a transposed copy (a separate
 // instantiation, so that profiles tell the two products apart); NT = 1 loads
-// the entry stream with the non-temporal hint.
-template <typename VT, int R, int J, int BWD, int NT>
+// the entry stream with the non-temporal hint.  CI is the stored column type:
+// int (global column) or uint16_t (tile-local column, col mod LDS_TILE).
+template <typename VT, int R, int J, int BWD, int NT, typename CI>
 __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, long long rows_per_wg,
                                                         const int64_t* __restrict__ rowptr,
                                                         const uint32_t* __restrict__ toff,
-                                                        const int* __restrict__ col,
+                                                        const CI* __restrict__ col,
                                                         const typename VT::T* __restrict__ val,
                                                         const float* __restrict__ x,
                                                         float* __restrict__ partial) {
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
     segment(rb + step, n0, n1);   // the next segment's bounds, requested before this one's entries
     float acc = 0.0f;
     for (long long base = s0; base < s1; base += (long long)L * J) {
-      int c[J];
+      CI c[J];
       float v[J];
       bool in[J];
 #pragma unroll
@@ -99,7 +100,8 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
 #pragma unroll
       for (int j = 0; j < J; j++) {
         // in-tile index; the clamp keeps LDS reads in range whatever the data
-        const unsigned li = std::min((unsigned)(c[j] - (int)tb), last);
+        const unsigned li =
+            std::min(sizeof(CI) == 2 ? (unsigned)c[j] : (unsigned)((int)c[j] - (int)tb), last);
         acc += in[j] ? v[j] * xs[li] : 0.0f;
       }
     }
@@ -107,6 +109,13 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
     if (ok && l == 0) partial[(long long)t * m + r] = acc;
   }
+}
+
+__global__ __launch_bounds__(256) void k_col16(const int* __restrict__ col, long long nnz,
+                                               uint16_t* __restrict__ col16) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (long long)gridDim.x * blockDim.x)
+    col16[k] = (uint16_t)(col[k] & (LDS_TILE - 1));
 }
 
 // y_r = alpha * scale_r * sum_t partial[t][r] + beta * y_r
@@ -154,29 +163,39 @@ __global__ __launch_bounds__(256) void k_lds_ratio(int m, int T, const float* __
   }
 }
 
-template <typename VT, int R, int J, int BWD, int NT>
+template <typename VT, int R, int J, int BWD, int NT, typename CI>
 static void set_lds_attr() {
   static bool done = false;
   if (!done) {
-    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD, NT>,
+    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD, NT, CI>,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               LDS_TILE * (int)sizeof(float));
     done = true;
   }
 }
 
-template <typename VT, int R, int J, int BWD, int NT>
-static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
-  set_lds_attr<VT, R, J, BWD, NT>();
+template <typename VT, int R, int J, int BWD, int NT, typename CI>
+static void launch_rows_lds_c(epet_ctx* ctx, const epet_matrix* A, const float* x, const CI* col) {
+  set_lds_attr<VT, R, J, BWD, NT, CI>();
   // each workgroup owns its CU's LDS; about four workgroups per CU in all, so
   // that tiles of unequal weight even out
   const long long G = std::max<long long>(1, (4ll * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
-  k_rows_lds<VT, R, J, BWD, NT><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
-                             ctx->stream>>>(
-      A->m, A->n, A->lds_T, rows_per_wg, A->rowptr, A->lds_toff, A->col,
+  k_rows_lds<VT, R, J, BWD, NT, CI><<<(unsigned)(ng * A->lds_T), LDS_BLOCK,
+                                      LDS_TILE * sizeof(float), ctx->stream>>>(
+      A->m, A->n, A->lds_T, rows_per_wg, A->rowptr, A->lds_toff, col,
       (const typename VT::T*)A->val, x, A->lds_partial);
+}
+
+// tile-local 16-bit columns when present (EPET_LDS_C16=0 keeps the int columns)
+template <typename VT, int R, int J, int BWD, int NT>
+static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
+  const char* e = getenv("EPET_LDS_C16");
+  if (A->lds_col16 && !(e && e[0] == '0'))
+    launch_rows_lds_c<VT, R, J, BWD, NT, uint16_t>(ctx, A, x, A->lds_col16);
+  else
+    launch_rows_lds_c<VT, R, J, BWD, NT, int>(ctx, A, x, A->col);
 }
 
 // Rows per wavefront from the mean segment length.  Entries are loaded with
@@ -237,6 +256,16 @@ int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
   }
   A->lds_T = T;
   launch_row_plan(ctx->stream, A->m, T, LDS_TILE, A->rowptr, A->col, A->lds_toff);
+  // tile-local 16-bit columns (optional: without them the int columns are read)
+  if (hipMalloc((void**)&A->lds_col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) ==
+      hipSuccess) {
+    if (A->nnz > 0)
+      k_col16<<<grid_for(A->nnz, 256, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(A->col, A->nnz,
+                                                                                   A->lds_col16);
+  } else {
+    (void)hipGetLastError();
+    A->lds_col16 = nullptr;
+  }
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }

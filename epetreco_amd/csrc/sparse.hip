@@ -354,6 +354,7 @@ epet_matrix::~epet_matrix() {
   if (xb_partial) (void)hipFree(xb_partial);
   if (lds_toff) (void)hipFree(lds_toff);
   if (lds_partial) (void)hipFree(lds_partial);
+  if (lds_col16) (void)hipFree(lds_col16);
   if (tr) epet_matrix_destroy(tr);
 }
 
