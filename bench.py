@@ -294,8 +294,8 @@ def main():
         "batched": batched,
         "roofline": {"bound": "hbm", "achieved": fwd_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": fwd_gbps / HBM_PEAK_GBPS,
-                     "traffic": measured_traffic(args.workload, "k_rows_lds<ValF32,4,16,0,0,unsignedshort>"),
-                     "kernel": "forward SpMV (k_rows_lds<ValF32,4,16,0,0,u16> + k_lds_reduce, timed together)",
+                     "traffic": measured_traffic(args.workload, "k_rows_lds<ValF32,4,16,0>"),
+                     "kernel": "forward SpMV (k_rows_lds<ValF32,4,16,0> + k_lds_reduce, timed together)",
                      "algorithmic_bytes": B_fwd, "avg_ms": fwd_ms},
         "cpu_baseline": cpu,
     }

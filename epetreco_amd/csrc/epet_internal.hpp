@@ -102,13 +102,16 @@ struct epet_matrix {
   // per-block partial products (XB_BLOCKS x m)
   uint32_t* xb_toff = nullptr;
   float* xb_partial = nullptr;
-  // LDS-tiled products: the gathered vector is cut into tiles of LDS_TILE
-  // entries; entry offsets of every tile boundary in every row (m x (lds_T+1))
-  // and the per-tile partial products (lds_T x m)
+  // LDS-tiled products: the gathered vector is cut into lds_T tiles of
+  // LDS_TILE entries, and the matrix is kept a second time in tile-major
+  // order: the entries of (tile t, row r) are lds_off[t m + r] .. lds_off[t m +
+  // r + 1] of lds_val (stored value type) and lds_col16 (tile-local column,
+  // col mod LDS_TILE); lds_partial holds the per-tile partial products (lds_T x m)
   int lds_T = 0;
-  uint32_t* lds_toff = nullptr;
+  int64_t* lds_off = nullptr;
+  void* lds_val = nullptr;
+  uint16_t* lds_col16 = nullptr;
   float* lds_partial = nullptr;
-  uint16_t* lds_col16 = nullptr;   // tile-local column of every entry (col mod LDS_TILE)
   bool lds_failed = false;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured

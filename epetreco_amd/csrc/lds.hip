@@ -8,8 +8,15 @@
 // by the per-matrix plan, since columns ascend within a row) with x read from
 // LDS.  The row's partial for tile t goes to partial[t m + r]; a reduction sums
 // the partials of every row in tile order (deterministic) and applies the row
-// scale, alpha/beta or the MLEM ratio.  HBM sees the matrix stream once, x
-// once per workgroup, and 8 bytes of plan and partial per (row, tile).
+// scale, alpha/beta or the MLEM ratio.
+//
+// The kernels read a tile-major copy of the matrix built once per matrix: the
+// entries of (tile t, row r) lie contiguously at lds_off[t m + r], rows of one
+// tile back to back, columns stored tile-local in 16 bits.  A cache line that
+// straddles two segments then belongs to neighbouring rows of the same
+// workgroup and is read from HBM once, and an FP32 entry costs 6 bytes.  HBM
+// sees the entry stream once, x once per workgroup, and 12 bytes of offset and
+// partial per (row, tile).
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -30,14 +37,11 @@ constexpr int LDS_WAVES = LDS_BLOCK / WAVE;
 
 // R rows per wavefront (L = 64/R lanes per row), J entries per lane per pass;
 // BWD = 1 marks the back projection over a transposed copy (a separate
-// instantiation, so that profiles tell the two products apart); NT = 1 loads
-// the entry stream with the non-temporal hint.  CI is the stored column type:
-// int (global column) or uint16_t (tile-local column, col mod LDS_TILE).
-template <typename VT, int R, int J, int BWD, int NT, typename CI>
+// instantiation, so that profiles tell the two products apart).
+template <typename VT, int R, int J, int BWD>
 __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, long long rows_per_wg,
-                                                        const int64_t* __restrict__ rowptr,
-                                                        const uint32_t* __restrict__ toff,
-                                                        const CI* __restrict__ col,
+                                                        const int64_t* __restrict__ off,
+                                                        const uint16_t* __restrict__ col,
                                                         const typename VT::T* __restrict__ val,
                                                         const float* __restrict__ x,
                                                         float* __restrict__ partial) {
@@ -59,16 +63,15 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
   const long long r1 = std::min<long long>((long long)m, r0 + rows_per_wg);
   const unsigned last = (unsigned)(cnt - 1);
   const long long step = (long long)LDS_WAVES * R;
+  const int64_t* offt = off + (long long)t * m;
   // segment [s0, s1) of row rbx + gi in tile t; empty past the range
   auto segment = [&](long long rbx, long long& s0, long long& s1) {
     s0 = 0;
     s1 = 0;
     const long long r = rbx + gi;
     if (r < r1) {
-      const long long b = rowptr[r];
-      const uint32_t* o = toff + r * (T + 1) + t;
-      s0 = b + o[0];
-      s1 = b + o[1];
+      s0 = offt[r];
+      s1 = offt[r + 1];
     }
   };
   long long rb = r0 + (long long)w * R;
@@ -81,43 +84,72 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
     segment(rb + step, n0, n1);   // the next segment's bounds, requested before this one's entries
     float acc = 0.0f;
     for (long long base = s0; base < s1; base += (long long)L * J) {
-      CI c[J];
+      uint16_t c[J];
       float v[J];
       bool in[J];
 #pragma unroll
       for (int j = 0; j < J; j++) {
+        // ordinary (cached) loads: a row group reads part of a cache line per
+        // load and the rest with the next one
         const long long k = base + l + (long long)j * L;
         in[j] = k < s1;
         const long long kc = in[j] ? k : s1 - 1;
-        if (NT) {
-          c[j] = __builtin_nontemporal_load(col + kc);
-          v[j] = VT::get(val, kc);
-        } else {
-          c[j] = col[kc];
-          v[j] = VT::get_cached(val, kc);
-        }
+        c[j] = col[kc];
+        v[j] = VT::get_cached(val, kc);
       }
 #pragma unroll
       for (int j = 0; j < J; j++) {
-        // in-tile index; the clamp keeps LDS reads in range whatever the data
-        const unsigned li =
-            std::min(sizeof(CI) == 2 ? (unsigned)c[j] : (unsigned)((int)c[j] - (int)tb), last);
+        // the clamp keeps LDS reads in range whatever the data
+        const unsigned li = std::min((unsigned)c[j], last);
         acc += in[j] ? v[j] * xs[li] : 0.0f;
       }
     }
 #pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
     if (ok && l == 0) partial[(long long)t * m + r] = acc;
   }
 }
 
-__global__ __launch_bounds__(256) void k_col16(const int* __restrict__ col, long long nnz,
-                                               uint16_t* __restrict__ col16) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
-       k += (long long)gridDim.x * blockDim.x)
-    col16[k] = (uint16_t)(col[k] & (LDS_TILE - 1));
+// ------------------------------------------------------------------ tile-major copy
+__global__ __launch_bounds__(256) void k_lds_counts(int m, int T, const uint32_t* __restrict__ toff,
+                                                    int* __restrict__ cnt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // i = t m + r
+  if (i >= (long long)m * T) return;
+  const int t = (int)(i / m);
+  const long long r = i % m;
+  const uint32_t* o = toff + r * (T + 1) + t;
+  cnt[i] = (int)(o[1] - o[0]);
 }
 
+__global__ void k_lds_set_last(int64_t* __restrict__ off, long long i, long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) off[i] = v;
+}
+
+// one wavefront per (tile, row) segment: copy values and tile-local columns
+template <typename T>
+__global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, const int64_t* __restrict__ rowptr,
+                                                  const uint32_t* __restrict__ toff,
+                                                  const int* __restrict__ col,
+                                                  const T* __restrict__ val,
+                                                  const int64_t* __restrict__ off,
+                                                  uint16_t* __restrict__ col16,
+                                                  T* __restrict__ val2) {
+  const int lane = lane_id();
+  const long long nw = (long long)gridDim.x * (256 / WAVE);
+  for (long long i = (long long)blockIdx.x * (256 / WAVE) + wave_id(); i < (long long)m * T_;
+       i += nw) {
+    const int t = (int)(i / m);
+    const long long r = i % m;
+    const long long src = rowptr[r] + toff[r * (T_ + 1) + t];
+    const long long dst = off[i], len = off[i + 1] - dst;
+    for (long long k = lane; k < len; k += WAVE) {
+      col16[dst + k] = (uint16_t)(col[src + k] & (LDS_TILE - 1));
+      val2[dst + k] = val[src + k];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reductions
 // y_r = alpha * scale_r * sum_t partial[t][r] + beta * y_r
 __global__ __launch_bounds__(256) void k_lds_reduce(int m, int T, const float* __restrict__ partial,
                                                     const float* __restrict__ rowscale,
@@ -163,63 +195,39 @@ __global__ __launch_bounds__(256) void k_lds_ratio(int m, int T, const float* __
   }
 }
 
-template <typename VT, int R, int J, int BWD, int NT, typename CI>
-static void set_lds_attr() {
-  static bool done = false;
-  if (!done) {
-    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD, NT, CI>,
+// ------------------------------------------------------------------ dispatch
+template <typename VT, int R, int J, int BWD>
+static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_rows_lds<VT, R, J, BWD>,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               LDS_TILE * (int)sizeof(float));
-    done = true;
+    attr = true;
   }
-}
-
-template <typename VT, int R, int J, int BWD, int NT, typename CI>
-static void launch_rows_lds_c(epet_ctx* ctx, const epet_matrix* A, const float* x, const CI* col) {
-  set_lds_attr<VT, R, J, BWD, NT, CI>();
   // each workgroup owns its CU's LDS; about four workgroups per CU in all, so
   // that tiles of unequal weight even out
   const long long G = std::max<long long>(1, (4ll * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
-  k_rows_lds<VT, R, J, BWD, NT, CI><<<(unsigned)(ng * A->lds_T), LDS_BLOCK,
-                                      LDS_TILE * sizeof(float), ctx->stream>>>(
-      A->m, A->n, A->lds_T, rows_per_wg, A->rowptr, A->lds_toff, col,
-      (const typename VT::T*)A->val, x, A->lds_partial);
+  k_rows_lds<VT, R, J, BWD><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
+                             ctx->stream>>>(A->m, A->n, A->lds_T, rows_per_wg, A->lds_off,
+                                            A->lds_col16, (const typename VT::T*)A->lds_val, x,
+                                            A->lds_partial);
 }
 
-// tile-local 16-bit columns when present (EPET_LDS_C16=0 keeps the int columns)
-template <typename VT, int R, int J, int BWD, int NT>
-static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
-  const char* e = getenv("EPET_LDS_C16");
-  if (A->lds_col16 && !(e && e[0] == '0'))
-    launch_rows_lds_c<VT, R, J, BWD, NT, uint16_t>(ctx, A, x, A->lds_col16);
-  else
-    launch_rows_lds_c<VT, R, J, BWD, NT, int>(ctx, A, x, A->col);
-}
-
-// Rows per wavefront from the mean segment length.  Entries are loaded with
-// ordinary (cached) loads: with R > 1 a row group's load covers only part of
-// a cache line, and the line's other part is read by the next load, which a
-// streaming hint would send back to HBM.  Measurement knobs: EPET_LDS_R (1, 2,
-// 4 or 16) forces R, EPET_LDS_NT=1 selects non-temporal loads.
-template <typename VT, int BWD, int NT>
-static void rows_lds_r(epet_ctx* ctx, const epet_matrix* A, const float* x, int R) {
-  if (R == 1) launch_rows_lds<VT, 1, 16, BWD, NT>(ctx, A, x);
-  else if (R == 2) launch_rows_lds<VT, 2, 16, BWD, NT>(ctx, A, x);
-  else if (R == 4) launch_rows_lds<VT, 4, 16, BWD, NT>(ctx, A, x);
-  else launch_rows_lds<VT, 16, 16, BWD, NT>(ctx, A, x);
-}
-
+// Rows per wavefront from the mean segment length (EPET_LDS_R = 1, 2, 4 or 16
+// forces it, for measurements).
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
   int R = seg >= 384.0 ? 2 : seg >= 48.0 ? 4 : 16;
   const char* er = getenv("EPET_LDS_R");
   if (er) R = atoi(er);
-  const char* en = getenv("EPET_LDS_NT");
-  if (en && en[0] == '1') rows_lds_r<VT, BWD, 1>(ctx, A, x, R);
-  else rows_lds_r<VT, BWD, 0>(ctx, A, x, R);
+  if (R == 1) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
+  else if (R == 2) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
+  else if (R == 4) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
+  else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
 }
 
 template <int BWD>
@@ -239,38 +247,70 @@ static bool lds_wanted(const epet_matrix* A) {
   return A->m > 0 && A->cols_strict && (double)A->nnz / A->m >= 64.0;
 }
 
-// Builds the plan and partial buffer once (allocates: call outside graph
-// capture).  Leaves the matrix without a plan when allocation fails.
-int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
-  if (A->lds_toff || A->lds_failed || !lds_wanted(A)) return EPET_OK;
-  const int T = (int)(((long long)A->n + LDS_TILE - 1) / LDS_TILE);
-  const long long nt = (long long)A->m * (T + 1);
-  if (T < 1 || hipMalloc((void**)&A->lds_toff, sizeof(uint32_t) * nt) != hipSuccess ||
-      hipMalloc((void**)&A->lds_partial, sizeof(float) * (size_t)A->m * T) != hipSuccess) {
-    (void)hipGetLastError();
-    if (A->lds_toff) (void)hipFree(A->lds_toff);
-    A->lds_toff = nullptr;
-    A->lds_partial = nullptr;
-    A->lds_failed = true;
-    return EPET_OK;
-  }
-  A->lds_T = T;
-  launch_row_plan(ctx->stream, A->m, T, LDS_TILE, A->rowptr, A->col, A->lds_toff);
-  // tile-local 16-bit columns (optional: without them the int columns are read)
-  if (hipMalloc((void**)&A->lds_col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) ==
-      hipSuccess) {
-    if (A->nnz > 0)
-      k_col16<<<grid_for(A->nnz, 256, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(A->col, A->nnz,
-                                                                                   A->lds_col16);
-  } else {
-    (void)hipGetLastError();
-    A->lds_col16 = nullptr;
-  }
-  EPET_HIP(hipGetLastError());
-  return EPET_OK;
+static void lds_release(epet_matrix* A) {
+  if (A->lds_off) (void)hipFree(A->lds_off);
+  if (A->lds_val) (void)hipFree(A->lds_val);
+  if (A->lds_col16) (void)hipFree(A->lds_col16);
+  if (A->lds_partial) (void)hipFree(A->lds_partial);
+  A->lds_off = nullptr;
+  A->lds_val = nullptr;
+  A->lds_col16 = nullptr;
+  A->lds_partial = nullptr;
+  A->lds_T = 0;
 }
 
-bool lds_ready(const epet_matrix* A) { return A->lds_toff != nullptr && lds_wanted(A); }
+// Builds the tile-major copy, its offsets and the partial buffer once
+// (allocates and synchronises: call outside graph capture).  Leaves the matrix
+// without them when an allocation fails; the other products then run.
+int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
+  if (A->lds_off || A->lds_failed || !lds_wanted(A)) return EPET_OK;
+  hipStream_t st = ctx->stream;
+  const int T = (int)(((long long)A->n + LDS_TILE - 1) / LDS_TILE);
+  const long long nseg = (long long)A->m * T;
+  const size_t vb = A->value_type == EPET_VAL_F32 ? 4 : 2;
+  uint32_t* toff = nullptr;
+  int* cnt = nullptr;
+  bool ok = T >= 1 &&
+            hipMalloc((void**)&toff, sizeof(uint32_t) * (size_t)A->m * (T + 1)) == hipSuccess &&
+            hipMalloc((void**)&cnt, sizeof(int) * (size_t)nseg) == hipSuccess &&
+            hipMalloc((void**)&A->lds_off, sizeof(int64_t) * (size_t)(nseg + 1)) == hipSuccess &&
+            hipMalloc(&A->lds_val, vb * (size_t)std::max(1ll, A->nnz)) == hipSuccess &&
+            hipMalloc((void**)&A->lds_col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) ==
+                hipSuccess &&
+            hipMalloc((void**)&A->lds_partial, sizeof(float) * (size_t)nseg) == hipSuccess;
+  int rc = EPET_OK;
+  if (ok) {
+    A->lds_T = T;
+    launch_row_plan(st, A->m, T, LDS_TILE, A->rowptr, A->col, toff);
+    k_lds_counts<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(A->m, T, toff, cnt);
+    long long tot = 0;
+    rc = exclusive_scan_i32_to_i64(ctx, cnt, A->lds_off, nseg, &tot);
+    if (rc == EPET_OK && tot != A->nnz) rc = fail(EPET_ERR_STATE, __FILE__, __LINE__,
+                                                  "tile-major copy: entry count mismatch");
+    if (rc == EPET_OK) {
+      k_lds_set_last<<<1, 64, 0, st>>>(A->lds_off, nseg, tot);
+      const unsigned g = (unsigned)std::min<long long>((nseg + 3) / 4, 16ll * ctx->num_cus);
+      if (vb == 4)
+        k_lds_fill<float><<<g, 256, 0, st>>>(A->m, T, A->rowptr, toff, A->col, (const float*)A->val,
+                                             A->lds_off, A->lds_col16, (float*)A->lds_val);
+      else
+        k_lds_fill<uint16_t><<<g, 256, 0, st>>>(A->m, T, A->rowptr, toff, A->col,
+                                                (const uint16_t*)A->val, A->lds_off,
+                                                A->lds_col16, (uint16_t*)A->lds_val);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "tile-major copy");
+    }
+  } else {
+    (void)hipGetLastError();
+    A->lds_failed = true;
+  }
+  if (toff) (void)hipFree(toff);
+  if (cnt) (void)hipFree(cnt);
+  if (!ok || rc != EPET_OK) lds_release(A);
+  return rc;
+}
+
+bool lds_ready(const epet_matrix* A) { return A->lds_off != nullptr && lds_wanted(A); }
 
 // y = alpha * diag(scale) A x + beta y (scale: the row scale when use_scale;
 // bwd: A is a transposed copy)

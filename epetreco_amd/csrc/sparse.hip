@@ -352,9 +352,10 @@ epet_matrix::~epet_matrix() {
   if (bp_partial) (void)hipFree(bp_partial);
   if (xb_toff) (void)hipFree(xb_toff);
   if (xb_partial) (void)hipFree(xb_partial);
-  if (lds_toff) (void)hipFree(lds_toff);
-  if (lds_partial) (void)hipFree(lds_partial);
+  if (lds_off) (void)hipFree(lds_off);
+  if (lds_val) (void)hipFree(lds_val);
   if (lds_col16) (void)hipFree(lds_col16);
+  if (lds_partial) (void)hipFree(lds_partial);
   if (tr) epet_matrix_destroy(tr);
 }
 

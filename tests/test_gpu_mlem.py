@@ -232,11 +232,11 @@ def test_lds_tiled_products_and_mlem(gpu_ctx_factory, oracle, storage, monkeypat
         c = A.spmv(OP_T, torch.as_tensor(e, device="cuda")).cpu().numpy()
         np.testing.assert_allclose(c, oracle.csrmv(1, m, n, 1.0, vq, rq, cq, e, 0.0,
                                                    np.zeros(n, np.float32)), rtol=2e-5, atol=1e-6)
-    # tile-local 16-bit columns (default) against the int columns
-    monkeypatch.setenv("EPET_LDS_C16", "0")
-    y_int = A.spmv(OP_N, torch.as_tensor(x, device="cuda")).cpu().numpy()
-    monkeypatch.delenv("EPET_LDS_C16")
-    np.testing.assert_allclose(y, y_int, rtol=1e-6, atol=1e-7)
+    # the tile-major LDS product against the row kernel over the original arrays
+    monkeypatch.setenv("EPET_LDS", "0")
+    y_rows = A.spmv(OP_N, torch.as_tensor(x, device="cuda")).cpu().numpy()
+    monkeypatch.delenv("EPET_LDS")
+    np.testing.assert_allclose(y, y_rows, rtol=1e-6, atol=1e-7)
     counts = rng.poisson(30.0, m).astype(np.float32)
     s = torch.as_tensor(np.bincount(cq, weights=vq, minlength=n).astype(np.float32), device="cuda")
     yy = torch.as_tensor(counts, device="cuda")
