@@ -33,6 +33,7 @@ SOURCES = {
     "mlem.hip": [],
     "mframe.hip": [],
     "lds.hip": [],
+    "mlds.hip": [],
 }
 
 

@@ -113,6 +113,15 @@ struct epet_matrix {
   uint16_t* lds_col16 = nullptr;
   float* lds_partial = nullptr;
   bool lds_failed = false;
+  // multi-frame LDS products: tile-major copy for tiles of LDS_TILE/mfl_P
+  // columns (P frames of each column in LDS), and partials (mfl_T x m x F)
+  int mfl_P = 0, mfl_T = 0;
+  int64_t* mfl_off = nullptr;
+  void* mfl_val = nullptr;
+  uint16_t* mfl_col16 = nullptr;
+  float* mfl_partial = nullptr;
+  long long mfl_partial_len = 0;
+  bool mfl_failed = false;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured
   int mf_F = 0, mf_fwd_R = 1, mf_fwd_P = 32, mf_bwd_R = 1, mf_bwd_P = 32;

@@ -127,7 +127,8 @@ __global__ void k_lds_set_last(int64_t* __restrict__ off, long long i, long long
 
 // one wavefront per (tile, row) segment: copy values and tile-local columns
 template <typename T>
-__global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, const int64_t* __restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, int mask,
+                                                  const int64_t* __restrict__ rowptr,
                                                   const uint32_t* __restrict__ toff,
                                                   const int* __restrict__ col,
                                                   const T* __restrict__ val,
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, const int64_t* 
     const long long src = rowptr[r] + toff[r * (T_ + 1) + t];
     const long long dst = off[i], len = off[i + 1] - dst;
     for (long long k = lane; k < len; k += WAVE) {
-      col16[dst + k] = (uint16_t)(col[src + k] & (LDS_TILE - 1));
+      col16[dst + k] = (uint16_t)(col[src + k] & mask);
       val2[dst + k] = val[src + k];
     }
   }
@@ -259,55 +260,86 @@ static void lds_release(epet_matrix* A) {
   A->lds_T = 0;
 }
 
-// Builds the tile-major copy, its offsets and the partial buffer once
-// (allocates and synchronises: call outside graph capture).  Leaves the matrix
-// without them when an allocation fails; the other products then run.
-int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
-  if (A->lds_off || A->lds_failed || !lds_wanted(A)) return EPET_OK;
+// Tile-major copy of A for column tiles of TV columns (a power of two up to
+// 65536): *T_out tiles, offsets (T m + 1), values and tile-local 16-bit
+// columns.  Allocates and synchronises.  On an allocation failure returns
+// EPET_OK with *off == nullptr.
+int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, int64_t** off_out,
+                     void** val_out, uint16_t** col_out) {
+  *off_out = nullptr;
+  *val_out = nullptr;
+  *col_out = nullptr;
   hipStream_t st = ctx->stream;
-  const int T = (int)(((long long)A->n + LDS_TILE - 1) / LDS_TILE);
+  const int T = (int)(((long long)A->n + TV - 1) / TV);
   const long long nseg = (long long)A->m * T;
   const size_t vb = A->value_type == EPET_VAL_F32 ? 4 : 2;
   uint32_t* toff = nullptr;
   int* cnt = nullptr;
+  int64_t* off = nullptr;
+  void* val = nullptr;
+  uint16_t* col16 = nullptr;
   bool ok = T >= 1 &&
             hipMalloc((void**)&toff, sizeof(uint32_t) * (size_t)A->m * (T + 1)) == hipSuccess &&
-            hipMalloc((void**)&cnt, sizeof(int) * (size_t)nseg) == hipSuccess &&
-            hipMalloc((void**)&A->lds_off, sizeof(int64_t) * (size_t)(nseg + 1)) == hipSuccess &&
-            hipMalloc(&A->lds_val, vb * (size_t)std::max(1ll, A->nnz)) == hipSuccess &&
-            hipMalloc((void**)&A->lds_col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) ==
-                hipSuccess &&
-            hipMalloc((void**)&A->lds_partial, sizeof(float) * (size_t)nseg) == hipSuccess;
+            hipMalloc((void**)&cnt, sizeof(int) * (size_t)std::max(1ll, nseg)) == hipSuccess &&
+            hipMalloc((void**)&off, sizeof(int64_t) * (size_t)(nseg + 1)) == hipSuccess &&
+            hipMalloc(&val, vb * (size_t)std::max(1ll, A->nnz)) == hipSuccess &&
+            hipMalloc((void**)&col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) == hipSuccess;
   int rc = EPET_OK;
   if (ok) {
-    A->lds_T = T;
-    launch_row_plan(st, A->m, T, LDS_TILE, A->rowptr, A->col, toff);
+    launch_row_plan(st, A->m, T, TV, A->rowptr, A->col, toff);
     k_lds_counts<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(A->m, T, toff, cnt);
     long long tot = 0;
-    rc = exclusive_scan_i32_to_i64(ctx, cnt, A->lds_off, nseg, &tot);
-    if (rc == EPET_OK && tot != A->nnz) rc = fail(EPET_ERR_STATE, __FILE__, __LINE__,
-                                                  "tile-major copy: entry count mismatch");
+    rc = exclusive_scan_i32_to_i64(ctx, cnt, off, nseg, &tot);
+    if (rc == EPET_OK && tot != A->nnz)
+      rc = fail(EPET_ERR_STATE, __FILE__, __LINE__, "tile-major copy: entry count mismatch");
     if (rc == EPET_OK) {
-      k_lds_set_last<<<1, 64, 0, st>>>(A->lds_off, nseg, tot);
-      const unsigned g = (unsigned)std::min<long long>((nseg + 3) / 4, 16ll * ctx->num_cus);
+      k_lds_set_last<<<1, 64, 0, st>>>(off, nseg, tot);
+      const unsigned g = (unsigned)std::max<long long>(
+          1, std::min<long long>((nseg + 3) / 4, 16ll * ctx->num_cus));
       if (vb == 4)
-        k_lds_fill<float><<<g, 256, 0, st>>>(A->m, T, A->rowptr, toff, A->col, (const float*)A->val,
-                                             A->lds_off, A->lds_col16, (float*)A->lds_val);
+        k_lds_fill<float><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, A->col,
+                                             (const float*)A->val, off, col16, (float*)val);
       else
-        k_lds_fill<uint16_t><<<g, 256, 0, st>>>(A->m, T, A->rowptr, toff, A->col,
-                                                (const uint16_t*)A->val, A->lds_off,
-                                                A->lds_col16, (uint16_t*)A->lds_val);
+        k_lds_fill<uint16_t><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, A->col,
+                                                (const uint16_t*)A->val, off, col16,
+                                                (uint16_t*)val);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "tile-major copy");
     }
   } else {
     (void)hipGetLastError();
-    A->lds_failed = true;
   }
   if (toff) (void)hipFree(toff);
   if (cnt) (void)hipFree(cnt);
-  if (!ok || rc != EPET_OK) lds_release(A);
-  return rc;
+  if (!ok || rc != EPET_OK) {
+    if (off) (void)hipFree(off);
+    if (val) (void)hipFree(val);
+    if (col16) (void)hipFree(col16);
+    return rc;
+  }
+  *T_out = T;
+  *off_out = off;
+  *val_out = val;
+  *col_out = col16;
+  return EPET_OK;
+}
+
+// Builds the tile-major copy and the partial buffer once (allocates and
+// synchronises: call outside graph capture).  Leaves the matrix without them
+// when an allocation fails; the other products then run.
+int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
+  if (A->lds_off || A->lds_failed || !lds_wanted(A)) return EPET_OK;
+  int T = 0;
+  EPET_TRY(build_tile_major(ctx, A, LDS_TILE, &T, &A->lds_off, &A->lds_val, &A->lds_col16));
+  if (A->lds_off && hipMalloc((void**)&A->lds_partial, sizeof(float) * (size_t)A->m * T) ==
+                        hipSuccess) {
+    A->lds_T = T;
+    return EPET_OK;
+  }
+  (void)hipGetLastError();
+  lds_release(A);
+  A->lds_failed = true;
+  return EPET_OK;
 }
 
 bool lds_ready(const epet_matrix* A) { return A->lds_off != nullptr && lds_wanted(A); }

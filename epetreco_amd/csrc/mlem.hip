@@ -38,6 +38,9 @@ int lds_fwd_ratio(epet_ctx* ctx, const epet_matrix* A, const float* x, const flo
                   int fold, const int* itp, double* loglik);
 int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
 int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
+int mfl_prepare(epet_ctx* ctx, epet_matrix* A, int F);
+bool mfl_ready(const epet_matrix* A, int F);
+int mfl_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
 
 // ------------------------------------------------------------------ element-wise
 __global__ __launch_bounds__(256) void k_divides(float* __restrict__ res, const float* __restrict__ a,
@@ -331,7 +334,7 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
   int rc;
   if (F > 1 && A->tr) {
     const MfArgs a{F, x, y, e, c, itp, loglik};
-    rc = mf_iteration(ctx, A, a);
+    rc = mfl_ready(A, F) ? mfl_iteration(ctx, A, a) : mf_iteration(ctx, A, a);
   } else {
     // single frame through the transposed copy: the ratio carries the row scale
     // and the back projection is a row-streaming product over the transpose
@@ -466,7 +469,11 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
     if (rc == EPET_OK && bp_use_transpose(A)) rc = lds_prepare(ctx, A->tr);
     if (rc == EPET_OK && !bp_use_transpose(A)) rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }
-  } else if (F > 1 && A->tr && A->mf_F != F) {
+  } else if (F > 1 && A->tr) {
+    rc = mfl_prepare(ctx, const_cast<epet_matrix*>(A), F);
+    if (rc != EPET_OK) { cleanup(); return rc; }
+  }
+  if (F > 1 && A->tr && !mfl_ready(A, F) && A->mf_F != F) {
     const MfArgs a{F, x, y, e, c, itp, nullptr};
     rc = mf_tune(ctx, const_cast<epet_matrix*>(A), a, 2);
     if (rc != EPET_OK) { cleanup(); return rc; }

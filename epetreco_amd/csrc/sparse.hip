@@ -356,6 +356,10 @@ epet_matrix::~epet_matrix() {
   if (lds_val) (void)hipFree(lds_val);
   if (lds_col16) (void)hipFree(lds_col16);
   if (lds_partial) (void)hipFree(lds_partial);
+  if (mfl_off) (void)hipFree(mfl_off);
+  if (mfl_val) (void)hipFree(mfl_val);
+  if (mfl_col16) (void)hipFree(mfl_col16);
+  if (mfl_partial) (void)hipFree(mfl_partial);
   if (tr) epet_matrix_destroy(tr);
 }
 

@@ -249,3 +249,38 @@ def test_lds_tiled_products_and_mlem(gpu_ctx_factory, oracle, storage, monkeypat
     assert rel_l2(x1.cpu().numpy(), x2.cpu().numpy().astype(np.float64)) < 1e-5
     x64 = oracle.mlem_f64(m, n, vq, rq, cq, counts, s.cpu().numpy(), np.full(n, 1.0 / n), 8)
     assert rel_l2(x1.cpu().numpy(), x64) < REL_L2
+
+
+@pytest.mark.parametrize("F,P", [(5, "4"), (16, "8"), (13, "8")])
+def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
+    """Multi-frame LDS-tiled MLEM (P frames per pass, ragged frame blocks when P
+    does not divide F) on long rows over 70,001 columns (several column tiles
+    each way), FP32 values: images after 6 iterations vs the gather kernels
+    (rel-L2 <= 1e-5) and vs the multi-frame FP64 oracle (<= 1e-4), and the
+    per-frame log-likelihood of the last iteration vs the gather kernels."""
+    from epetreco_amd.api import Matrix
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(F)
+    m, n, avg = 5000, 70001, 250
+    lens = rng.poisson(avg, m)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    val = rng.random(rp[-1]).astype(np.float32)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    A.attach_transpose()
+    Y = rng.poisson(20.0, (m, F)).astype(np.float32)
+    s = torch.as_tensor(np.bincount(col, weights=val, minlength=n).astype(np.float32), device="cuda")
+    Yd = torch.as_tensor(Y, device="cuda")
+    monkeypatch.setenv("EPET_MFL_P", P)
+    X1 = torch.full((n, F), 1.0 / n, device="cuda")
+    ll1, _ = ctx.mlem(A, Yd, s, X1, 6, use_graph=True)
+    monkeypatch.setenv("EPET_MFL", "0")
+    X2 = torch.full((n, F), 1.0 / n, device="cuda")
+    ll2, _ = ctx.mlem(A, Yd, s, X2, 6, use_graph=True)
+    monkeypatch.delenv("EPET_MFL")
+    monkeypatch.delenv("EPET_MFL_P")
+    X64 = oracle.mlem_f64_frames(m, n, val, rp, col, Y, s.cpu().numpy(), np.full((n, F), 1.0 / n), 6)
+    for f in range(F):
+        assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
+        assert rel_l2(X1[:, f].cpu().numpy(), X64[:, f]) < REL_L2
+    np.testing.assert_allclose(ll1[-1], ll2[-1], rtol=1e-7)
