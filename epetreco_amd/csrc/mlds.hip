@@ -112,11 +112,13 @@ __global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, i
     for (int f = 0; f < P; f++)
 #pragma unroll
       for (int o = L / 2; o > 0; o >>= 1) acc[f] += __shfl_xor(acc[f], o, L);
-    // lane l < P of the row group stores frame l (selects, no dynamic indexing)
-    float mine = 0.0f;
+    // lane l of the row group stores the frames f with f mod L = l; empty
+    // segments store nothing (the reductions skip them by their offsets)
+    if (ok && s1 > s0) {
 #pragma unroll
-    for (int f = 0; f < P; f++) mine = (l == f) ? acc[f] : mine;
-    if (ok && l < P && f0 + l < F) partial[((long long)t * m + r) * F + f0 + l] = mine;
+      for (int f = 0; f < P; f++)
+        if (f % L == l && f0 + f < F) partial[((long long)t * m + r) * F + f0 + f] = acc[f];
+    }
   }
 }
 
@@ -124,6 +126,7 @@ __global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, i
 // yhat = 0; times the row scale when fold = 1), per-frame FP64
 // log-likelihood summed per workgroup in LDS, one global atomic per frame.
 __global__ __launch_bounds__(256) void k_mfl_ratio(int m, int F, int T,
+                                                   const int64_t* __restrict__ off,
                                                    const float* __restrict__ partial,
                                                    const float* __restrict__ rowscale,
                                                    const float* __restrict__ Y,
@@ -137,10 +140,13 @@ __global__ __launch_bounds__(256) void k_mfl_ratio(int m, int F, int T,
   const long long stride_t = (long long)m * F;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
        i += (long long)gridDim.x * blockDim.x) {
-    float s = 0.0f;
-    for (int t = 0; t < T; t++) s += partial[(long long)t * stride_t + i];
     const long long r = i / F;
     const int f = (int)(i % F);
+    float s = 0.0f;
+    for (int t = 0; t < T; t++) {
+      const int64_t* o = off + (long long)t * m + r;
+      if (o[1] > o[0]) s += partial[(long long)t * stride_t + i];
+    }
     const float sc = rowscale ? rowscale[r] : 1.0f;
     const float yh = s * sc;
     const float yi = Y[i];
@@ -153,13 +159,20 @@ __global__ __launch_bounds__(256) void k_mfl_ratio(int m, int F, int T,
     unsafeAtomicAdd(&loglik[(long long)(*itp) * F + threadIdx.x], s_ll[threadIdx.x]);
 }
 
-// Back-projection reduction: C = sum_t partial (rows of the transposed copy)
-__global__ __launch_bounds__(256) void k_mfl_sum(long long tot, int T, const float* __restrict__ partial,
+// Back-projection reduction: C = sum_t partial over the non-empty segments
+// (rows of the transposed copy)
+__global__ __launch_bounds__(256) void k_mfl_sum(int m, int F, int T, const int64_t* __restrict__ off,
+                                                 const float* __restrict__ partial,
                                                  float* __restrict__ C) {
+  const long long tot = (long long)m * F;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= tot) return;
+  const long long r = i / F;
   float s = 0.0f;
-  for (int t = 0; t < T; t++) s += partial[(long long)t * tot + i];
+  for (int t = 0; t < T; t++) {
+    const int64_t* o = off + (long long)t * m + r;
+    if (o[1] > o[0]) s += partial[(long long)t * tot + i];
+  }
   C[i] = s;
 }
 
@@ -286,11 +299,12 @@ int mfl_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a) {
   const epet_matrix* At = A->tr;
   mfl_product<0>(ctx, A, a.F, a.x);
   k_mfl_ratio<<<grid_for((long long)A->m * a.F, 256, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(
-      A->m, a.F, A->mfl_T, A->mfl_partial, A->rowscale, a.y, a.e, A->rowscale ? 1 : 0, a.itp,
-      a.loglik);
+      A->m, a.F, A->mfl_T, A->mfl_off, A->mfl_partial, A->rowscale, a.y, a.e, A->rowscale ? 1 : 0,
+      a.itp, a.loglik);
   mfl_product<1>(ctx, At, a.F, a.e);
   const long long tot = (long long)At->m * a.F;
-  k_mfl_sum<<<grid_for(tot, 256, 1ll << 31), 256, 0, ctx->stream>>>(tot, At->mfl_T, At->mfl_partial,
+  k_mfl_sum<<<grid_for(tot, 256, 1ll << 31), 256, 0, ctx->stream>>>(At->m, a.F, At->mfl_T,
+                                                                    At->mfl_off, At->mfl_partial,
                                                                     a.c);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
