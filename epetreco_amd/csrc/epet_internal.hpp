@@ -73,6 +73,27 @@ struct epet_ctx {
   int num_cus = 256;
 };
 
+// Column order of a tile-major copy.  nx = 0: the matrix's own column order.
+// Otherwise the columns are voxels (ix ny + iy) nz + iz of an nx x ny x nz
+// grid, renumbered y-major, (iy nx + ix) nz + iz, so that tiles are slabs of
+// y-planes: a channel's line keeps its y within a few planes, so its row
+// meets few tiles.
+struct ColMap {
+  int nx = 0, ny = 0, nz = 0;
+};
+__host__ __device__ inline long long colmap_fwd(long long c, const ColMap& mp) {
+  if (mp.nx == 0) return c;
+  const long long iz = c % mp.nz, q = c / mp.nz;
+  const long long iy = q % mp.ny, ix = q / mp.ny;
+  return (iy * mp.nx + ix) * mp.nz + iz;
+}
+__host__ __device__ inline long long colmap_inv(long long pc, const ColMap& mp) {
+  if (mp.nx == 0) return pc;
+  const long long iz = pc % mp.nz, q = pc / mp.nz;
+  const long long ix = q % mp.nx, iy = q / mp.nx;
+  return (ix * mp.ny + iy) * mp.nz + iz;
+}
+
 struct epet_matrix {
   int m = 0, n = 0;
   long long nnz = 0;
@@ -112,6 +133,7 @@ struct epet_matrix {
   void* lds_val = nullptr;
   uint16_t* lds_col16 = nullptr;
   float* lds_partial = nullptr;
+  ColMap lds_map;
   bool lds_failed = false;
   // multi-frame LDS products: tile-major copy for tiles of LDS_TILE/mfl_P
   // columns (P frames of each column in LDS), and partials (mfl_T x m x F)
@@ -121,6 +143,7 @@ struct epet_matrix {
   uint16_t* mfl_col16 = nullptr;
   float* mfl_partial = nullptr;
   long long mfl_partial_len = 0;
+  ColMap mfl_map;
   bool mfl_failed = false;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured
@@ -134,6 +157,8 @@ constexpr int XB_BLOCKS = 8;
 // LDS tile of the LDS-tiled row products: 32768 floats = 128 KB of the 160 KB
 // of LDS per workgroup on gfx950
 constexpr int LDS_TILE = 32768;
+
+
 
 namespace epet {
 

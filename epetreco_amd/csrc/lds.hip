@@ -44,18 +44,23 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
                                                         const uint16_t* __restrict__ col,
                                                         const typename VT::T* __restrict__ val,
                                                         const float* __restrict__ x,
-                                                        float* __restrict__ partial) {
+                                                        float* __restrict__ partial, ColMap mp) {
   extern __shared__ float xs[];
   constexpr int L = WAVE / R;
   const int t = blockIdx.x % T;
   const long long g = blockIdx.x / T;
   const long long tb = (long long)t * LDS_TILE;
   const int cnt = (int)std::min<long long>(LDS_TILE, (long long)n - tb);
-  // tile copy: 16-byte loads when the tile start is 16-byte aligned
-  const int c4 = ((reinterpret_cast<uintptr_t>(x + tb) & 15) == 0) ? (cnt >> 2) : 0;
-  for (int i = threadIdx.x; i < c4; i += LDS_BLOCK)
-    reinterpret_cast<float4*>(xs)[i] = reinterpret_cast<const float4*>(x + tb)[i];
-  for (int i = (c4 << 2) + threadIdx.x; i < cnt; i += LDS_BLOCK) xs[i] = x[tb + i];
+  if (mp.nx == 0) {
+    // tile copy: 16-byte loads when the tile start is 16-byte aligned
+    const int c4 = ((reinterpret_cast<uintptr_t>(x + tb) & 15) == 0) ? (cnt >> 2) : 0;
+    for (int i = threadIdx.x; i < c4; i += LDS_BLOCK)
+      reinterpret_cast<float4*>(xs)[i] = reinterpret_cast<const float4*>(x + tb)[i];
+    for (int i = (c4 << 2) + threadIdx.x; i < cnt; i += LDS_BLOCK) xs[i] = x[tb + i];
+  } else {
+    // renumbered columns: tile entry i is column colmap_inv(tb + i) of x
+    for (int i = threadIdx.x; i < cnt; i += LDS_BLOCK) xs[i] = x[colmap_inv(tb + i, mp)];
+  }
   __syncthreads();
   const int lane = lane_id(), gi = lane / L, l = lane % L;
   const int w = __builtin_amdgcn_readfirstlane(wave_id());
@@ -111,6 +116,64 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
 }
 
 // ------------------------------------------------------------------ tile-major copy
+// Stable counting sort of every row's entries by the voxel's y index (one
+// wavefront per row, per-wave bucket counters in LDS): rows ascending in
+// (ix, iy, iz) become ascending in (iy, ix, iz), i.e. in the y-major column
+// colmap_fwd(c).  Writes the renumbered columns and the values in that order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ysort_rows(int m, const int64_t* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const T* __restrict__ val, ColMap mp,
+                                                    int* __restrict__ colp, T* __restrict__ valp) {
+  extern __shared__ int bucket[];                 // (256 / WAVE) x ny
+  const int lane = lane_id();
+  const int w = wave_id();
+  int* cnt = bucket + w * mp.ny;
+  const long long nw = (long long)gridDim.x * (256 / WAVE);
+  for (long long r = (long long)blockIdx.x * (256 / WAVE) + w; r < m; r += nw) {
+    const long long b = rowptr[r], e = rowptr[r + 1];
+    for (int i = lane; i < mp.ny; i += WAVE) cnt[i] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (long long k = b + lane; k < e; k += WAVE) atomicAdd(&cnt[(col[k] / mp.nz) % mp.ny], 1);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      int run = 0;
+      for (int i = 0; i < mp.ny; i++) {
+        const int c = cnt[i];
+        cnt[i] = run;
+        run += c;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (long long base = b; base < e; base += WAVE) {
+      const long long k = base + lane;
+      bool act = k < e;
+      const int c = act ? col[k] : 0;
+      const int iy = act ? (c / mp.nz) % mp.ny : -1;
+      // lanes grouped by y, each group placed after the earlier entries of its
+      // bucket in lane (= entry) order
+      unsigned long long left = __ballot(act);
+      while (left) {
+        const int lead = __ffsll((long long)left) - 1;
+        const int key = __shfl(iy, lead);
+        const unsigned long long mask = __ballot(act && iy == key);
+        const int start = cnt[key];
+        if (act && iy == key) {
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          colp[b + start + rank] = (int)colmap_fwd(c, mp);
+          valp[b + start + rank] = val[k];
+          act = false;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == lead) cnt[key] = start + __popcll(mask);
+        __builtin_amdgcn_wave_barrier();
+        left &= ~mask;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 __global__ __launch_bounds__(256) void k_lds_counts(int m, int T, const uint32_t* __restrict__ toff,
                                                     int* __restrict__ cnt) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // i = t m + r
@@ -214,7 +277,7 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
   k_rows_lds<VT, R, J, BWD><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
                              ctx->stream>>>(A->m, A->n, A->lds_T, rows_per_wg, A->lds_off,
                                             A->lds_col16, (const typename VT::T*)A->lds_val, x,
-                                            A->lds_partial);
+                                            A->lds_partial, A->lds_map);
 }
 
 // Rows per wavefront from the mean segment length (EPET_LDS_R = 1, 2, 4 or 16
@@ -264,12 +327,39 @@ static void lds_release(epet_matrix* A) {
 // 65536): *T_out tiles, offsets (T m + 1), values and tile-local 16-bit
 // columns.  Allocates and synchronises.  On an allocation failure returns
 // EPET_OK with *off == nullptr.
-int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, int64_t** off_out,
-                     void** val_out, uint16_t** col_out) {
+int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, ColMap mp, int* T_out,
+                     int64_t** off_out, void** val_out, uint16_t** col_out) {
   *off_out = nullptr;
   *val_out = nullptr;
   *col_out = nullptr;
   hipStream_t st = ctx->stream;
+  const int* scol = A->col;
+  const void* sval = A->val;
+  int* colp = nullptr;
+  void* valp = nullptr;
+  if (mp.nx > 0) {
+    // renumbered columns: rows re-sorted into y-major column order first
+    const size_t vb0 = A->value_type == EPET_VAL_F32 ? 4 : 2;
+    if (hipMalloc((void**)&colp, sizeof(int) * (size_t)std::max(1ll, A->nnz)) != hipSuccess ||
+        hipMalloc(&valp, vb0 * (size_t)std::max(1ll, A->nnz)) != hipSuccess) {
+      (void)hipGetLastError();
+      if (colp) (void)hipFree(colp);
+      return EPET_OK;
+    }
+    const unsigned g = (unsigned)std::max<long long>(
+        1, std::min<long long>(((long long)A->m + 3) / 4, 16ll * ctx->num_cus));
+    const size_t lds = sizeof(int) * (256 / WAVE) * (size_t)mp.ny;
+    if (vb0 == 4)
+      k_ysort_rows<float><<<g, 256, lds, st>>>(A->m, A->rowptr, A->col, (const float*)A->val, mp,
+                                               colp, (float*)valp);
+    else
+      k_ysort_rows<uint16_t><<<g, 256, lds, st>>>(A->m, A->rowptr, A->col,
+                                                  (const uint16_t*)A->val, mp, colp,
+                                                  (uint16_t*)valp);
+    EPET_HIP(hipGetLastError());
+    scol = colp;
+    sval = valp;
+  }
   const int T = (int)(((long long)A->n + TV - 1) / TV);
   const long long nseg = (long long)A->m * T;
   const size_t vb = A->value_type == EPET_VAL_F32 ? 4 : 2;
@@ -286,7 +376,7 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, in
             hipMalloc((void**)&col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) == hipSuccess;
   int rc = EPET_OK;
   if (ok) {
-    launch_row_plan(st, A->m, T, TV, A->rowptr, A->col, toff);
+    launch_row_plan(st, A->m, T, TV, A->rowptr, scol, toff);
     k_lds_counts<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(A->m, T, toff, cnt);
     long long tot = 0;
     rc = exclusive_scan_i32_to_i64(ctx, cnt, off, nseg, &tot);
@@ -297,11 +387,11 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, in
       const unsigned g = (unsigned)std::max<long long>(
           1, std::min<long long>((nseg + 3) / 4, 16ll * ctx->num_cus));
       if (vb == 4)
-        k_lds_fill<float><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, A->col,
-                                             (const float*)A->val, off, col16, (float*)val);
+        k_lds_fill<float><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, scol,
+                                             (const float*)sval, off, col16, (float*)val);
       else
-        k_lds_fill<uint16_t><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, A->col,
-                                                (const uint16_t*)A->val, off, col16,
+        k_lds_fill<uint16_t><<<g, 256, 0, st>>>(A->m, T, TV - 1, A->rowptr, toff, scol,
+                                                (const uint16_t*)sval, off, col16,
                                                 (uint16_t*)val);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "tile-major copy");
@@ -311,6 +401,8 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, in
   }
   if (toff) (void)hipFree(toff);
   if (cnt) (void)hipFree(cnt);
+  if (colp) (void)hipFree(colp);
+  if (valp) (void)hipFree(valp);
   if (!ok || rc != EPET_OK) {
     if (off) (void)hipFree(off);
     if (val) (void)hipFree(val);
@@ -324,13 +416,37 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, in
   return EPET_OK;
 }
 
+int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
+
 // Builds the tile-major copy and the partial buffer once (allocates and
-// synchronises: call outside graph capture).  Leaves the matrix without them
+// synchronises: call outside graph capture).  voxel_columns: the columns are
+// image voxels (a forward projector), so they may be renumbered y-major.  Leaves the matrix without them
 // when an allocation fails; the other products then run.
+// y-major column order when the columns are the voxels of the context's grid
+// (a smaller LDS footprint per row; EPET_LDS_MAP=0 keeps the matrix's order)
+ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A) {
+  ColMap mp;
+  const char* e = getenv("EPET_LDS_MAP");
+  if (e && e[0] == '0') return mp;
+  if (!ctx->have_geo) return mp;
+  const int* gn = ctx->geo.gn;
+  if ((long long)gn[0] * gn[1] * gn[2] != A->n || gn[1] > 4096) return mp;
+  mp.nx = gn[0];
+  mp.ny = gn[1];
+  mp.nz = gn[2];
+  return mp;
+}
+
 int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
+  return lds_prepare_map(ctx, A, false);
+}
+
+int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns) {
   if (A->lds_off || A->lds_failed || !lds_wanted(A)) return EPET_OK;
   int T = 0;
-  EPET_TRY(build_tile_major(ctx, A, LDS_TILE, &T, &A->lds_off, &A->lds_val, &A->lds_col16));
+  const ColMap mp = voxel_columns ? voxel_map(ctx, A) : ColMap{};
+  EPET_TRY(build_tile_major(ctx, A, LDS_TILE, mp, &T, &A->lds_off, &A->lds_val, &A->lds_col16));
+  A->lds_map = mp;
   if (A->lds_off && hipMalloc((void**)&A->lds_partial, sizeof(float) * (size_t)A->m * T) ==
                         hipSuccess) {
     A->lds_T = T;

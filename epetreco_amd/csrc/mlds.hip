@@ -24,8 +24,9 @@
 
 namespace epet {
 
-int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, int* T_out, int64_t** off_out,
-                     void** val_out, uint16_t** col_out);
+int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, ColMap mp, int* T_out,
+                     int64_t** off_out, void** val_out, uint16_t** col_out);
+ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A);
 
 constexpr int MFL_BLOCK = 1024;
 constexpr int MFL_WAVES = MFL_BLOCK / WAVE;
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, i
                                                    const uint16_t* __restrict__ col,
                                                    const typename VT::T* __restrict__ val,
                                                    const float* __restrict__ X,
-                                                   float* __restrict__ partial) {
+                                                   float* __restrict__ partial, ColMap mp) {
   extern __shared__ float xs[];
   constexpr int TV = LDS_TILE / P;
   constexpr int L = WAVE / R;
@@ -49,10 +50,11 @@ __global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, i
   const long long tb = (long long)t * TV;
   const int cnt = (int)std::min<long long>(TV, (long long)n - tb);
   const int f0 = fb * P;
-  // tile: xs[v P + f] = X[(tb + v) F + f0 + f], 0 for frames past F
+  // tile: xs[v P + f] = X[c F + f0 + f] for the tile's v-th column c (in the
+// copy's column order), 0 for frames past F
   for (int i = threadIdx.x; i < cnt * P; i += MFL_BLOCK) {
     const int v = i / P, f = i % P;
-    xs[i] = (f0 + f < F) ? X[(tb + v) * F + f0 + f] : 0.0f;
+    xs[i] = (f0 + f < F) ? X[colmap_inv(tb + v, mp) * F + f0 + f] : 0.0f;
   }
   __syncthreads();
   const int lane = lane_id(), gi = lane / L, l = lane % L;
@@ -193,7 +195,7 @@ static void launch_mfl(epet_ctx* ctx, const epet_matrix* A, int F, const float* 
   k_mfl<VT, P, R, J, BWD><<<(unsigned)(ng * units), MFL_BLOCK, LDS_TILE * sizeof(float),
                            ctx->stream>>>(A->m, A->n, F, A->mfl_T, FB, rows_per_wg, A->mfl_off,
                                           A->mfl_col16, (const typename VT::T*)A->mfl_val, X,
-                                          A->mfl_partial);
+                                          A->mfl_partial, A->mfl_map);
 }
 
 template <typename VT, int BWD>
@@ -246,13 +248,16 @@ static void mfl_release(epet_matrix* A) {
 // Tile-major copy for F frames and a partial buffer large enough (allocates
 // and synchronises: call outside graph capture).  Without memory the matrix
 // is left as it was and the gather kernels run.
-static int mfl_prepare_one(epet_ctx* ctx, epet_matrix* A, int F) {
+static int mfl_prepare_one(epet_ctx* ctx, epet_matrix* A, int F, bool voxel_columns) {
   if (A->mfl_failed || !mfl_wanted(A)) return EPET_OK;
   const int P = mfl_frames_per_pass(F);
   if (A->mfl_off && A->mfl_P != P) mfl_release(A);
   if (!A->mfl_off) {
     int T = 0;
-    EPET_TRY(build_tile_major(ctx, A, LDS_TILE / P, &T, &A->mfl_off, &A->mfl_val, &A->mfl_col16));
+    const ColMap mp = voxel_columns ? voxel_map(ctx, A) : ColMap{};
+    EPET_TRY(build_tile_major(ctx, A, LDS_TILE / P, mp, &T, &A->mfl_off, &A->mfl_val,
+                              &A->mfl_col16));
+    A->mfl_map = mp;
     if (!A->mfl_off) {
       A->mfl_failed = true;
       return EPET_OK;
@@ -279,8 +284,8 @@ static int mfl_prepare_one(epet_ctx* ctx, epet_matrix* A, int F) {
 
 int mfl_prepare(epet_ctx* ctx, epet_matrix* A, int F) {
   if (!A->tr || F < 2) return EPET_OK;
-  EPET_TRY(mfl_prepare_one(ctx, A, F));
-  EPET_TRY(mfl_prepare_one(ctx, A->tr, F));
+  EPET_TRY(mfl_prepare_one(ctx, A, F, true));
+  EPET_TRY(mfl_prepare_one(ctx, A->tr, F, false));
   return EPET_OK;
 }
 

@@ -33,6 +33,7 @@ bool bp_use_transpose(const epet_matrix* A);
 int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
                         float beta, float* y);
 int lds_prepare(epet_ctx* ctx, epet_matrix* A);
+int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
 bool lds_ready(const epet_matrix* A);
 int lds_fwd_ratio(epet_ctx* ctx, const epet_matrix* A, const float* x, const float* y, float* e,
                   int fold, const int* itp, double* loglik);
@@ -465,7 +466,7 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
 
   // back-projection plan (allocates) and multi-frame tuning before any capture
   if (F == 1) {
-    rc = lds_prepare(ctx, const_cast<epet_matrix*>(A));
+    rc = lds_prepare_map(ctx, const_cast<epet_matrix*>(A), true);
     if (rc == EPET_OK && bp_use_transpose(A)) rc = lds_prepare(ctx, A->tr);
     if (rc == EPET_OK && !bp_use_transpose(A)) rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }

@@ -470,6 +470,7 @@ void xb_prepare(epet_ctx* ctx, epet_matrix* At) {
 }
 
 int lds_prepare(epet_ctx* ctx, epet_matrix* A);
+int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
 bool lds_ready(const epet_matrix* A);
 int lds_spmv(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha, float beta,
              float* y, bool use_scale, bool bwd);
@@ -499,7 +500,7 @@ static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alph
                          float beta, float* y) {
   const double avg = A->m ? (double)A->nnz / A->m : 0.0;
   if (op == EPET_OP_N) {
-    EPET_TRY(lds_prepare(ctx, const_cast<epet_matrix*>(A)));
+    EPET_TRY(lds_prepare_map(ctx, const_cast<epet_matrix*>(A), true));
     if (lds_ready(A)) return lds_spmv(ctx, A, x, alpha, beta, y, true, false);
   }
   const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel

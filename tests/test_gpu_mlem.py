@@ -284,3 +284,35 @@ def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
         assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
         assert rel_l2(X1[:, f].cpu().numpy(), X64[:, f]) < REL_L2
     np.testing.assert_allclose(ll1[-1], ll2[-1], rtol=1e-7)
+
+
+def test_voxel_renumbered_tiles(gpu_ctx_factory, monkeypatch):
+    """Tile-major copies in y-major voxel order (the default for matrices over
+    the context's grid): the projector workload's forward product (single
+    frame) and a 16-frame MLEM match the copies in the matrix's own column
+    order and the kernels without LDS copies."""
+    from epetreco_amd.api import OP_N
+    ctx = gpu_ctx_factory("projector", seed=0)
+    A = ctx.sm_build(None, m=ctx.setup.nchannels)
+    rng = np.random.default_rng(3)
+    x = torch.as_tensor(rng.random(A.n).astype(np.float32), device="cuda")
+    y_map = A.spmv(OP_N, x).cpu().numpy()
+    monkeypatch.setenv("EPET_LDS", "0")
+    y_rows = A.spmv(OP_N, x).cpu().numpy()
+    monkeypatch.delenv("EPET_LDS")
+    np.testing.assert_allclose(y_map, y_rows, rtol=1e-6, atol=1e-7)
+    A.attach_transpose()
+    F = 16
+    Y = torch.as_tensor(rng.poisson(5.0, (A.m, F)).astype(np.float32), device="cuda")
+    s = A.colsum_f64().float()
+    X1 = torch.full((A.n, F), 1.0 / A.n, device="cuda")
+    ctx.mlem(A, Y, s, X1, 4, use_graph=True)
+    from epetreco_amd.api import Matrix
+    rp, col, val, _ = A.to_torch()
+    B = Matrix.from_csr(ctx, rp, col, val, A.n).attach_transpose()
+    monkeypatch.setenv("EPET_LDS_MAP", "0")
+    X2 = torch.full((A.n, F), 1.0 / A.n, device="cuda")
+    ctx.mlem(B, Y, s, X2, 4, use_graph=True)
+    monkeypatch.delenv("EPET_LDS_MAP")
+    for f in range(F):
+        assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
