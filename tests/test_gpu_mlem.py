@@ -287,11 +287,12 @@ def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
 
 
 def test_voxel_renumbered_tiles(gpu_ctx_factory, monkeypatch):
-    """Tile-major copies in y-major voxel order (the default for matrices over
-    the context's grid): the projector workload's forward product (single
+    """Tile-major copies in y-major voxel order (EPET_LDS_MAP=1, for matrices
+    over the context's grid): the projector workload's forward product (single
     frame) and a 16-frame MLEM match the copies in the matrix's own column
     order and the kernels without LDS copies."""
     from epetreco_amd.api import OP_N
+    monkeypatch.setenv("EPET_LDS_MAP", "1")
     ctx = gpu_ctx_factory("projector", seed=0)
     A = ctx.sm_build(None, m=ctx.setup.nchannels)
     rng = np.random.default_rng(3)
