@@ -280,8 +280,8 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
                                             A->lds_partial, A->lds_map);
 }
 
-// Rows per wavefront from the mean segment length (EPET_LDS_R = 1, 2, 4 or 16
-// forces it, for measurements).
+// Rows per wavefront from the mean segment length (EPET_LDS_R = 1, 2, 4, 8 or
+// 16 forces it, for measurements; R = 8 takes 24 entries per lane per pass).
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
@@ -291,6 +291,7 @@ static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* 
   if (R == 1) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
   else if (R == 2) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
   else if (R == 4) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
+  else if (R == 8) launch_rows_lds<VT, 8, 24, BWD>(ctx, A, x);
   else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
 }
 
