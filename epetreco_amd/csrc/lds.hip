@@ -269,9 +269,13 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
                               LDS_TILE * (int)sizeof(float));
     attr = true;
   }
-  // each workgroup owns its CU's LDS; about four workgroups per CU in all, so
-  // that tiles of unequal weight even out
-  const long long G = std::max<long long>(1, (4ll * ctx->num_cus + A->lds_T - 1) / A->lds_T);
+  // each workgroup owns its CU's LDS; several workgroups per CU in all, so
+  // that tiles and row ranges of unequal weight even out: 8 for the forward
+  // and 16 for the back projection were fastest at 52^3 (EPET_LDS_WG
+  // overrides them, for measurements)
+  const char* ew = getenv("EPET_LDS_WG");
+  const long long per_cu = ew ? std::max(1, atoi(ew)) : (BWD ? 16 : 8);
+  const long long G = std::max<long long>(1, (per_cu * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
   k_rows_lds<VT, R, J, BWD><<<(unsigned)(ng * A->lds_T), LDS_BLOCK, LDS_TILE * sizeof(float),
