@@ -202,14 +202,21 @@ static void launch_mfl(epet_ctx* ctx, const epet_matrix* A, int F, const float* 
                                           A->mfl_partial, A->mfl_map);
 }
 
+// rows per wavefront from the mean segment length (EPET_MFL_R = 4, 8 or 16
+// forces it, for measurements)
 template <typename VT, int BWD>
 static void mfl_dispatch(epet_ctx* ctx, const epet_matrix* A, int F, const float* X) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->mfl_T);
+  int R = seg >= 48.0 ? 4 : 16;
+  const char* er = getenv("EPET_MFL_R");
+  if (er) R = atoi(er);
   if (A->mfl_P == 8) {
-    if (seg >= 48.0) launch_mfl<VT, 8, 4, 16, BWD>(ctx, A, F, X);
+    if (R == 4) launch_mfl<VT, 8, 4, 16, BWD>(ctx, A, F, X);
+    else if (R == 8) launch_mfl<VT, 8, 8, 8, BWD>(ctx, A, F, X);
     else launch_mfl<VT, 8, 16, 8, BWD>(ctx, A, F, X);
   } else {
-    if (seg >= 48.0) launch_mfl<VT, 4, 4, 16, BWD>(ctx, A, F, X);
+    if (R == 4) launch_mfl<VT, 4, 4, 16, BWD>(ctx, A, F, X);
+    else if (R == 8) launch_mfl<VT, 4, 8, 8, BWD>(ctx, A, F, X);
     else launch_mfl<VT, 4, 16, 8, BWD>(ctx, A, F, X);
   }
 }
