@@ -220,6 +220,17 @@ class Context:
         check(lib().epet_sum(self._h, _p(x), x.numel(), ctypes.byref(out)))
         return out.value
 
+    def getChords(self, rays: torch.Tensor, nChords: int):
+        """Siddon chords of rays (n x 6) on the context's grid: (chords (n,
+        nChords), voxelIds (n, nChords, 3), counts (n), -1 = capacity short)."""
+        rays = rays.contiguous()
+        n = rays.numel() // 6
+        ch = torch.zeros(n, max(nChords, 1), dtype=torch.float32, device=self.device)
+        vid = torch.zeros(n, max(nChords, 1), 3, dtype=torch.int32, device=self.device)
+        cnt = torch.zeros(n, dtype=torch.int32, device=self.device)
+        check(lib().epet_getChords(self._h, _p(rays), n, int(nChords), _p(ch), _p(vid), _p(cnt)))
+        return ch, vid, cnt
+
     def mlem_tune(self, A: "Matrix", frames: int):
         """Autotune the multi-frame kernels for `frames`; returns
         ((R_fwd, P_fwd, R_bwd, P_bwd), (ms_fwd, ms_bwd))."""

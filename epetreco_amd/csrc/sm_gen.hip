@@ -465,6 +465,115 @@ static int generate_rows(epet_ctx* ctx, const Geo& g, const int* list, int m,
 
 int matrix_alloc(epet_matrix** out, int m, int n, long long nnz, int value_type);
 
+
+// ------------------------------------------------------------------ Siddon chords
+// getChords (ChordsCalc_lowlevel.hpp:173-357) for a batch of rays, one thread
+// per ray, in the reference's single-precision operation order (this file is
+// built without contraction and with IEEE division and square root), with the
+// reference's quirks: axes the ray does not cross still take part in the
+// crossing comparison (:312), aDimnext starts at aDimmin + aDimup (:242), and a
+// chord past the exit plane can be written (:292).  Ray r's chords go to
+// chords[r nChords + k], its voxel (ix, iy, iz) to voxelIds[3 (r nChords + k)];
+// counts[r] = chords written, or -1 when nChords is too small (a check the
+// reference lacks).  The loop makes progress on every pass (one plane is
+// crossed and alpha grows by aDimup > 0) and is capped at nChords + 1 passes.
+__device__ __forceinline__ void masked_extreme3(float* out, int* good, const float a[3],
+                                                const int b[3], bool want_max) {
+  int g = 0;
+  float v = 0.0f;
+  for (int d = 0; d < 3; d++) {
+    if (!b[d]) continue;
+    if (!g) {
+      v = a[d];
+      g = 1;
+    } else {
+      v = want_max ? fmaxf(v, a[d]) : fminf(v, a[d]);
+    }
+  }
+  *out = v;
+  *good = g;
+}
+
+__global__ __launch_bounds__(256) void k_get_chords(const float* __restrict__ rays, int nrays,
+                                                    Geo g, int nChords,
+                                                    float* __restrict__ chords,
+                                                    int* __restrict__ voxelIds,
+                                                    int* __restrict__ counts) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  float ray[6];
+  for (int i = 0; i < 6; i++) ray[i] = rays[r * 6 + i];
+  const float* O = g.go;
+  const float* D = g.gd;
+  const int* N = g.gn;
+  float* ch = chords + r * nChords;
+  int* vid = voxelIds + r * nChords * 3;
+  float admin[3], admax[3];
+  int crosses[3];
+  for (int d = 0; d < 3; d++) {   // getAlphaDimmin/max :484-533, alpha_from_id :421-434
+    float divisor = ray[d + 3] - ray[d];
+    divisor += (float)(divisor == 0.0f);
+    const float a0 = (O[d] + 0.0f * D[d] - ray[d]) / divisor;
+    const float a1 = (O[d] + (float)N[d] * D[d] - ray[d]) / divisor;
+    admin[d] = fminf(a0, a1);
+    admax[d] = fmaxf(a0, a1);
+  }
+  int goodPar = 1;                 // getCrossesPlanes :373-399
+  for (int d = 0; d < 3; d++) {
+    crosses[d] = (ray[d + 3] - ray[d] != 0.0f);
+    goodPar &= (crosses[d] ||
+                (!crosses[d] && ray[d] >= O[d] && ray[d] < O[d] + D[d] * (float)N[d]));
+  }
+  for (int d = 0; d < 3; d++) crosses[d] &= goodPar;
+  float aMin, aMax;
+  int aMinGood, aMaxGood;
+  masked_extreme3(&aMin, &aMinGood, admin, crosses, true);
+  masked_extreme3(&aMax, &aMaxGood, admax, crosses, false);
+  aMinGood &= ((double)aMin >= 0. && (double)aMin <= 1.);   // :208-209
+  aMaxGood &= ((double)aMax >= 0. && (double)aMax <= 1.);
+  if (aMin > aMax || !aMinGood || !aMaxGood) {              // :218
+    counts[r] = 0;
+    return;
+  }
+  float aup[3], anext[3];
+  int idup[3], id[3];
+  for (int d = 0; d < 3; d++) {   // getAlphaDimup :612-629, getIdDimup :642-654
+    const float dv = ray[d + 3] - ray[d];
+    float divisor = sqrtf(dv * dv);
+    divisor += (float)(int)(divisor == 0.0f);
+    aup[d] = D[d] / divisor;
+    idup[d] = (int)(ray[d] < ray[d + 3]) - (int)(ray[d] >= ray[d + 3]);
+    anext[d] = admin[d] + aup[d];
+  }
+  float aNext;
+  int aNextExists;
+  masked_extreme3(&aNext, &aNextExists, anext, crosses, false);
+  for (int d = 0; d < 3; d++)     // phi_from_alpha :453-465
+    id[d] = (int)floorf((ray[d] + (0.5f * (aMin + aNext)) * (ray[d + 3] - ray[d]) - O[d]) / D[d]);
+  float aCurr = aMin;
+  const float dx = ray[3] - ray[0], dy = ray[4] - ray[1], dz = ray[5] - ray[2];
+  const float length = sqrtf((dx * dx + dy * dy) + dz * dz);   // getLength :588-599
+  int k = 0;
+  for (int pass = 0; aCurr < aMax && pass <= nChords; pass++) {   // :292-356
+    masked_extreme3(&aNext, &aNextExists, anext, crosses, false);
+    for (int d = 0; d < 3; d++) {
+      if (anext[d] == aNext) {
+        if (k >= nChords) {
+          counts[r] = -1;
+          return;
+        }
+        ch[k] = 1.0f * (anext[d] - aCurr) * length;
+        for (int w = 0; w < 3; w++) vid[3 * k + w] = id[w];
+        k++;
+        aCurr = anext[d];
+        anext[d] += aup[d];
+        id[d] += idup[d];
+      }
+    }
+  }
+  counts[r] = aCurr < aMax ? -1 : k;
+}
+
 }  // namespace epet
 
 using namespace epet;
@@ -696,6 +805,21 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
   A->val = val;
   A->device = ctx->device;
   *out = A;
+  return EPET_OK;
+}
+
+
+int epet_getChords(epet_ctx* ctx, const float* rays_dev, int nrays, int nChords,
+                   float* chords_dev, int* voxelIds_dev, int* counts_dev) {
+  EPET_CHECK(ctx && (nrays == 0 || (rays_dev && chords_dev && voxelIds_dev && counts_dev)),
+             EPET_ERR_ARG, "null argument");
+  EPET_CHECK(nrays >= 0 && nChords >= 0, EPET_ERR_ARG, "negative size");
+  Geo g;
+  EPET_TRY(ctx_geo(ctx, &g));
+  if (nrays > 0)
+    k_get_chords<<<grid_for(nrays, 256, 1ll << 31), 256, 0, ctx->stream>>>(
+        rays_dev, nrays, g, nChords, chords_dev, voxelIds_dev, counts_dev);
+  EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
 

@@ -103,6 +103,17 @@ int epet_memcpy_d2d(epet_ctx* ctx, void* dst, const void* src, size_t bytes);
  * Host replay: identical bits. */
 int epet_rays(epet_ctx* ctx, const int* cnl_dev, int n, float* rays_dev);
 
+/* Siddon chords of a batch of rays (getChords, ChordsCalc_lowlevel.hpp:173-357)
+ * on the context's grid: rays_dev holds nrays x 6 floats (start, end).  Ray r
+ * writes its chords to chords_dev[r*nChords + k] and the voxel (ix, iy, iz)
+ * of each chord to voxelIds_dev[3*(r*nChords + k)]; counts_dev[r] = chords
+ * written, or -1 when nChords is too small (a check the reference lacks).
+ * The reference's quirks are kept: a ray with an endpoint inside the grid
+ * yields no chords (:208-218), non-crossed axes take part in the crossing
+ * test (:312), the first alpha step starts at aDimmin + aDimup (:242). */
+int epet_getChords(epet_ctx* ctx, const float* rays_dev, int nrays, int nChords,
+                   float* chords_dev, int* voxelIds_dev, int* counts_dev);
+
 /* ---- workqueue (getWorkqueue.hpp:129-223) ----------------------------------- */
 /* Scans (list entry, voxel) pairs from the cursor (*listId, *vxlId) in
  * lexicographic order and emits candidate pairs (getWorkqueue.hpp:50-78),

@@ -144,3 +144,56 @@ def test_system_matrix_calculation_chain(gpu_ctx_factory, oracle):
     np.testing.assert_array_equal(ecsr.cpu().numpy().astype(np.int64), orp)
     np.testing.assert_array_equal(vxl.cpu().numpy(), ocol)
     assert np.array_equal(val.cpu().numpy(), oval)
+
+
+def test_get_chords_matches_oracle(gpu_ctx_factory, oracle):
+    """Device Siddon chords (getChords, ChordsCalc_lowlevel.hpp:173-357) are
+    bit-exact with the oracle's restatement, quirks included, on: detector rays
+    of 200 channels, 300 random rays between points outside the grid,
+    axis-parallel rays, and rays with an endpoint inside the grid (no chords);
+    a short capacity gives -1."""
+    import ctypes
+    import torch
+    ctx = gpu_ctx_factory("small", seed=0)
+    g = ctx.grid
+    rng = np.random.default_rng(9)
+    cnl = torch.as_tensor(rng.choice(ctx.setup.nchannels, 200, replace=False).astype(np.int32),
+                          device="cuda")
+    det = ctx.rays(cnl).reshape(-1, 6).cpu().numpy()[::4]          # a quarter of the rays
+    c = np.array([g.ox + g.nx * g.dx / 2, g.oy + g.ny * g.dy / 2, g.oz + g.nz * g.dz / 2])
+    u = rng.normal(size=(300, 2, 3))
+    u /= np.linalg.norm(u, axis=2, keepdims=True)
+    rnd = (c + 0.06 * u).reshape(300, 6)
+    axis = []
+    for d in range(3):
+        p = c + rng.uniform(-0.02, 0.02, 3)
+        a, b = p.copy(), p.copy()
+        a[d], b[d] = c[d] - 0.1, c[d] + 0.1
+        axis.append(np.concatenate([a, b]))
+        axis.append(np.concatenate([b, a]))
+    inside = np.concatenate([c + 0.001, c + 0.2])[None, :]
+    rays = np.concatenate([det, rnd, np.array(axis), inside]).astype(np.float32)
+    n = len(rays)
+    cap = 64
+    ch, vid, cnt = ctx.getChords(torch.as_tensor(rays, device="cuda"), cap)
+    ch, vid, cnt = ch.cpu().numpy(), vid.cpu().numpy(), cnt.cpu().numpy()
+    L = oracle.lib()
+    O = np.array([g.ox, g.oy, g.oz], np.float32)
+    D = np.array([g.dx, g.dy, g.dz], np.float32)
+    N = np.array([g.nx, g.ny, g.nz], np.int32)
+    nonzero = 0
+    for r in range(n):
+        och = np.zeros(cap, np.float32)
+        oid = np.zeros(3 * cap, np.int32)
+        k = L.eo_get_chords(oracle.ptr(och, ctypes.c_float), oracle.ptr(oid, ctypes.c_int), cap,
+                            oracle.ptr(rays[r], ctypes.c_float), oracle.ptr(O, ctypes.c_float),
+                            oracle.ptr(D, ctypes.c_float), oracle.ptr(N, ctypes.c_int))
+        assert cnt[r] == k, r
+        if k > 0:
+            nonzero += 1
+            assert np.array_equal(ch[r, :k], och[:k]), r
+            assert np.array_equal(vid[r, :k].reshape(-1), oid[:3 * k]), r
+    assert nonzero > 200 and cnt[-1] == 0
+    _, _, cnt5 = ctx.getChords(torch.as_tensor(rays, device="cuda"), 5)
+    cnt5 = cnt5.cpu().numpy()
+    assert np.all((cnt5 == -1) == (cnt > 5)) and np.all(cnt5[cnt <= 5] == cnt[cnt <= 5])
