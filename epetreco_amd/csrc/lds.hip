@@ -294,7 +294,11 @@ static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* 
   if (er) R = atoi(er);
   if (R == 1) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
   else if (R == 2) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
-  else if (R == 4) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
+  else if (R == 4) {
+    const char* ej = getenv("EPET_LDS_J");   // 12: 192 entries per row and pass
+    if (ej && atoi(ej) == 12) launch_rows_lds<VT, 4, 12, BWD>(ctx, A, x);
+    else launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
+  }
   else if (R == 8) launch_rows_lds<VT, 8, 24, BWD>(ctx, A, x);
   else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
 }
