@@ -137,7 +137,7 @@ struct epet_matrix {
   bool lds_failed = false;
   // multi-frame LDS products: tile-major copy for tiles of LDS_TILE/mfl_P
   // columns (P frames of each column in LDS), and partials (mfl_T x m x F)
-  int mfl_P = 0, mfl_T = 0;
+  int mfl_P = 0, mfl_T = 0, mfl_tile = 0;   // tile: LDS floats per workgroup
   int64_t* mfl_off = nullptr;
   void* mfl_val = nullptr;
   uint16_t* mfl_col16 = nullptr;

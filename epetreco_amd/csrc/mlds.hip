@@ -31,16 +31,17 @@ ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A);
 constexpr int MFL_BLOCK = 1024;
 constexpr int MFL_WAVES = MFL_BLOCK / WAVE;
 
-template <typename VT, int P, int R, int J, int BWD>
-__global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, int FB,
-                                                   long long rows_per_wg,
-                                                   const int64_t* __restrict__ off,
-                                                   const uint16_t* __restrict__ col,
-                                                   const typename VT::T* __restrict__ val,
-                                                   const float* __restrict__ X,
-                                                   float* __restrict__ partial, ColMap mp) {
+// TF: LDS floats per workgroup (32768 = 128 KB, one workgroup per CU; 16384 =
+// 64 KB, two per CU with at most 64 VGPRs)
+template <typename VT, int P, int R, int J, int BWD, int TF>
+__device__ __forceinline__ void mfl_body(int m, int n, int F, int T, int FB, long long rows_per_wg,
+                                         const int64_t* __restrict__ off,
+                                         const uint16_t* __restrict__ col,
+                                         const typename VT::T* __restrict__ val,
+                                         const float* __restrict__ X,
+                                         float* __restrict__ partial, ColMap mp) {
   extern __shared__ float xs[];
-  constexpr int TV = LDS_TILE / P;
+  constexpr int TV = TF / P;
   constexpr int L = WAVE / R;
   static_assert(P % 4 == 0, "frames per pass: multiple of 4");
   const int t = blockIdx.x % T;
@@ -124,6 +125,26 @@ __global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, i
   }
 }
 
+template <typename VT, int P, int R, int J, int BWD, int TF>
+__global__ __launch_bounds__(MFL_BLOCK) void k_mfl(int m, int n, int F, int T, int FB,
+                                                   long long rows_per_wg,
+                                                   const int64_t* __restrict__ off,
+                                                   const uint16_t* __restrict__ col,
+                                                   const typename VT::T* __restrict__ val,
+                                                   const float* __restrict__ X,
+                                                   float* __restrict__ partial, ColMap mp) {
+  mfl_body<VT, P, R, J, BWD, TF>(m, n, F, T, FB, rows_per_wg, off, col, val, X, partial, mp);
+}
+
+// the 64 KB-tile variant: two workgroups per CU, so at most 64 VGPRs
+template <typename VT, int P, int R, int J, int BWD, int TF>
+__global__ __launch_bounds__(MFL_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_mfl2(
+    int m, int n, int F, int T, int FB, long long rows_per_wg, const int64_t* __restrict__ off,
+    const uint16_t* __restrict__ col, const typename VT::T* __restrict__ val,
+    const float* __restrict__ X, float* __restrict__ partial, ColMap mp) {
+  mfl_body<VT, P, R, J, BWD, TF>(m, n, F, T, FB, rows_per_wg, off, col, val, X, partial, mp);
+}
+
 // Forward reduction: yhat = scale_r sum_t partial, E = Y / yhat (0 where
 // yhat = 0; times the row scale when fold = 1), per-frame FP64
 // log-likelihood summed per workgroup in LDS, one global atomic per frame.
@@ -178,13 +199,13 @@ __global__ __launch_bounds__(256) void k_mfl_sum(int m, int F, int T, const int6
   C[i] = s;
 }
 
-template <typename VT, int P, int R, int J, int BWD>
-static void launch_mfl(epet_ctx* ctx, const epet_matrix* A, int F, const float* X) {
+template <typename VT, int P, int R, int J, int BWD, int TF>
+static void launch_mfl_t(epet_ctx* ctx, const epet_matrix* A, int F, const float* X) {
+  auto kern = TF == LDS_TILE ? k_mfl<VT, P, R, J, BWD, TF> : k_mfl2<VT, P, R, J, BWD, TF>;
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_mfl<VT, P, R, J, BWD>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              LDS_TILE * (int)sizeof(float));
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              TF * (int)sizeof(float));
     attr = true;
   }
   const int FB = (F + P - 1) / P;
@@ -196,10 +217,15 @@ static void launch_mfl(epet_ctx* ctx, const epet_matrix* A, int F, const float* 
   const long long G = std::max<long long>(1, (per_cu * ctx->num_cus + units - 1) / units);
   const long long rows_per_wg = std::max<long long>(MFL_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
-  k_mfl<VT, P, R, J, BWD><<<(unsigned)(ng * units), MFL_BLOCK, LDS_TILE * sizeof(float),
-                           ctx->stream>>>(A->m, A->n, F, A->mfl_T, FB, rows_per_wg, A->mfl_off,
+  kern<<<(unsigned)(ng * units), MFL_BLOCK, TF * sizeof(float), ctx->stream>>>(A->m, A->n, F, A->mfl_T, FB, rows_per_wg, A->mfl_off,
                                           A->mfl_col16, (const typename VT::T*)A->mfl_val, X,
                                           A->mfl_partial, A->mfl_map);
+}
+
+template <typename VT, int P, int R, int J, int BWD>
+static void launch_mfl(epet_ctx* ctx, const epet_matrix* A, int F, const float* X) {
+  if (A->mfl_tile == LDS_TILE / 2) launch_mfl_t<VT, P, R, J, BWD, LDS_TILE / 2>(ctx, A, F, X);
+  else launch_mfl_t<VT, P, R, J, BWD, LDS_TILE>(ctx, A, F, X);
 }
 
 // rows per wavefront from the mean segment length (EPET_MFL_R = 4, 8 or 16
@@ -252,7 +278,7 @@ static void mfl_release(epet_matrix* A) {
   A->mfl_val = nullptr;
   A->mfl_col16 = nullptr;
   A->mfl_partial = nullptr;
-  A->mfl_P = A->mfl_T = 0;
+  A->mfl_P = A->mfl_T = A->mfl_tile = 0;
   A->mfl_partial_len = 0;
 }
 
@@ -262,13 +288,17 @@ static void mfl_release(epet_matrix* A) {
 static int mfl_prepare_one(epet_ctx* ctx, epet_matrix* A, int F, bool voxel_columns) {
   if (A->mfl_failed || !mfl_wanted(A)) return EPET_OK;
   const int P = mfl_frames_per_pass(F);
-  if (A->mfl_off && A->mfl_P != P) mfl_release(A);
+  // EPET_MFL_TILE_KB=64: 64 KB tiles, two workgroups per CU (measurements)
+  const char* et = getenv("EPET_MFL_TILE_KB");
+  const int tile = (et && atoi(et) == 64) ? LDS_TILE / 2 : LDS_TILE;
+  if (A->mfl_off && (A->mfl_P != P || A->mfl_tile != tile)) mfl_release(A);
   if (!A->mfl_off) {
     int T = 0;
     const ColMap mp = voxel_columns ? voxel_map(ctx, A) : ColMap{};
-    EPET_TRY(build_tile_major(ctx, A, LDS_TILE / P, mp, &T, &A->mfl_off, &A->mfl_val,
+    EPET_TRY(build_tile_major(ctx, A, tile / P, mp, &T, &A->mfl_off, &A->mfl_val,
                               &A->mfl_col16));
     A->mfl_map = mp;
+    A->mfl_tile = tile;
     if (!A->mfl_off) {
       A->mfl_failed = true;
       return EPET_OK;
@@ -306,7 +336,8 @@ bool mfl_ready(const epet_matrix* A, int F) {
   auto ok = [F](const epet_matrix* M) {
     return M && M->mfl_off && M->mfl_partial_len >= (long long)M->mfl_T * M->m * F;
   };
-  return F >= 2 && ok(A) && ok(A->tr) && A->mfl_P == A->tr->mfl_P;
+  return F >= 2 && ok(A) && ok(A->tr) && A->mfl_P == A->tr->mfl_P &&
+         A->mfl_tile == A->tr->mfl_tile;
 }
 
 // Forward + back projection of one multi-frame MLEM iteration (capturable once
