@@ -145,13 +145,9 @@ __global__ __launch_bounds__(MFL_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)
   mfl_body<VT, P, R, J, BWD, TF>(m, n, F, T, FB, rows_per_wg, off, col, val, X, partial, mp);
 }
 
-// Reductions: one wavefront per row (lanes = frames, F <= 64); the row's tile
-// offsets are read once per wavefront, empty tiles are skipped as a whole,
-// and a non-empty tile's F partials are read as one coalesced run.
-//
-// Forward: yhat = scale_r sum_t partial, E = Y / yhat (0 where yhat = 0; times
-// the row scale when fold = 1), per-frame FP64 log-likelihood summed per
-// workgroup in LDS, one global atomic per frame.
+// Forward reduction: yhat = scale_r sum_t partial, E = Y / yhat (0 where
+// yhat = 0; times the row scale when fold = 1), per-frame FP64
+// log-likelihood summed per workgroup in LDS, one global atomic per frame.
 __global__ __launch_bounds__(256) void k_mfl_ratio(int m, int F, int T,
                                                    const int64_t* __restrict__ off,
                                                    const float* __restrict__ partial,
@@ -163,45 +159,44 @@ __global__ __launch_bounds__(256) void k_mfl_ratio(int m, int F, int T,
   __shared__ double s_ll[64];
   if (threadIdx.x < 64) s_ll[threadIdx.x] = 0.0;
   __syncthreads();
-  const int f = lane_id();
-  const long long nw = (long long)gridDim.x * (256 / WAVE);
-  double ll = 0.0;
-  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
+  const long long tot = (long long)m * F;
+  const long long stride_t = (long long)m * F;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / F;
+    const int f = (int)(i % F);
     float s = 0.0f;
     for (int t = 0; t < T; t++) {
       const int64_t* o = off + (long long)t * m + r;
-      if (o[1] > o[0] && f < F) s += partial[((long long)t * m + r) * F + f];
+      if (o[1] > o[0]) s += partial[(long long)t * stride_t + i];
     }
-    if (f < F) {
-      const float sc = rowscale ? rowscale[r] : 1.0f;
-      const float yh = s * sc;
-      const float yi = Y[r * F + f];
-      const float ev = yh > 0.0f ? yi / yh : 0.0f;
-      E[r * F + f] = fold ? ev * sc : ev;
-      if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
-    }
+    const float sc = rowscale ? rowscale[r] : 1.0f;
+    const float yh = s * sc;
+    const float yi = Y[i];
+    const float ev = yh > 0.0f ? yi / yh : 0.0f;
+    E[i] = fold ? ev * sc : ev;
+    if (yh > 0.0f) atomicAdd(&s_ll[f], (double)yi * log((double)yh) - (double)yh);
   }
-  if (f < F) atomicAdd(&s_ll[f], ll);
   __syncthreads();
   if (loglik && threadIdx.x < F)
     unsafeAtomicAdd(&loglik[(long long)(*itp) * F + threadIdx.x], s_ll[threadIdx.x]);
 }
 
-// Back projection: C = sum_t partial over the non-empty segments (rows of the
-// transposed copy)
+// Back-projection reduction: C = sum_t partial over the non-empty segments
+// (rows of the transposed copy)
 __global__ __launch_bounds__(256) void k_mfl_sum(int m, int F, int T, const int64_t* __restrict__ off,
                                                  const float* __restrict__ partial,
                                                  float* __restrict__ C) {
-  const int f = lane_id();
-  const long long nw = (long long)gridDim.x * (256 / WAVE);
-  for (long long r = (long long)blockIdx.x * (256 / WAVE) + wave_id(); r < m; r += nw) {
-    float s = 0.0f;
-    for (int t = 0; t < T; t++) {
-      const int64_t* o = off + (long long)t * m + r;
-      if (o[1] > o[0] && f < F) s += partial[((long long)t * m + r) * F + f];
-    }
-    if (f < F) C[r * F + f] = s;
+  const long long tot = (long long)m * F;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tot) return;
+  const long long r = i / F;
+  float s = 0.0f;
+  for (int t = 0; t < T; t++) {
+    const int64_t* o = off + (long long)t * m + r;
+    if (o[1] > o[0]) s += partial[(long long)t * tot + i];
   }
+  C[i] = s;
 }
 
 template <typename VT, int P, int R, int J, int BWD, int TF>
@@ -350,12 +345,14 @@ bool mfl_ready(const epet_matrix* A, int F) {
 int mfl_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a) {
   const epet_matrix* At = A->tr;
   mfl_product<0>(ctx, A, a.F, a.x);
-  k_mfl_ratio<<<grid_for((long long)A->m, 256 / WAVE, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(
+  k_mfl_ratio<<<grid_for((long long)A->m * a.F, 256, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(
       A->m, a.F, A->mfl_T, A->mfl_off, A->mfl_partial, A->rowscale, a.y, a.e, A->rowscale ? 1 : 0,
       a.itp, a.loglik);
   mfl_product<1>(ctx, At, a.F, a.e);
-  k_mfl_sum<<<grid_for((long long)At->m, 256 / WAVE, 16ll * ctx->num_cus), 256, 0, ctx->stream>>>(
-      At->m, a.F, At->mfl_T, At->mfl_off, At->mfl_partial, a.c);
+  const long long tot = (long long)At->m * a.F;
+  k_mfl_sum<<<grid_for(tot, 256, 1ll << 31), 256, 0, ctx->stream>>>(At->m, a.F, At->mfl_T,
+                                                                    At->mfl_off, At->mfl_partial,
+                                                                    a.c);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
