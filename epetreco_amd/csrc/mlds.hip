@@ -262,8 +262,11 @@ static bool mfl_wanted(const epet_matrix* A) {
   return A->m > 0 && A->cols_strict && (double)A->nnz / A->m >= 16.0;
 }
 
-// frames per pass for F frames (EPET_MFL_P = 4 or 8 forces it)
-static int mfl_frames_per_pass(int F) {
+// frames per pass for F frames (EPET_MFL_P = 4 or 8 forces it; EPET_MFL_P_BWD
+// for the transposed copy alone, for measurements)
+static int mfl_frames_per_pass(int F, bool bwd) {
+  const char* eb = bwd ? getenv("EPET_MFL_P_BWD") : nullptr;
+  if (eb) return atoi(eb) == 4 ? 4 : 8;
   const char* e = getenv("EPET_MFL_P");
   if (e) return atoi(e) == 4 ? 4 : 8;
   return F >= 8 ? 8 : 4;
@@ -287,7 +290,7 @@ static void mfl_release(epet_matrix* A) {
 // is left as it was and the gather kernels run.
 static int mfl_prepare_one(epet_ctx* ctx, epet_matrix* A, int F, bool voxel_columns) {
   if (A->mfl_failed || !mfl_wanted(A)) return EPET_OK;
-  const int P = mfl_frames_per_pass(F);
+  const int P = mfl_frames_per_pass(F, !voxel_columns);
   // EPET_MFL_TILE_KB=64: 64 KB tiles, two workgroups per CU (measurements)
   const char* et = getenv("EPET_MFL_TILE_KB");
   const int tile = (et && atoi(et) == 64) ? LDS_TILE / 2 : LDS_TILE;
@@ -336,8 +339,7 @@ bool mfl_ready(const epet_matrix* A, int F) {
   auto ok = [F](const epet_matrix* M) {
     return M && M->mfl_off && M->mfl_partial_len >= (long long)M->mfl_T * M->m * F;
   };
-  return F >= 2 && ok(A) && ok(A->tr) && A->mfl_P == A->tr->mfl_P &&
-         A->mfl_tile == A->tr->mfl_tile;
+  return F >= 2 && ok(A) && ok(A->tr);
 }
 
 // Forward + back projection of one multi-frame MLEM iteration (capturable once
