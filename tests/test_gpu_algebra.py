@@ -205,3 +205,29 @@ def test_invalid_csr_rejected(gpu_ctx_factory):
     col = torch.tensor([0, 5, 1], dtype=torch.int32)     # 5 >= n
     with pytest.raises(EpetError):
         Matrix.from_csr(ctx, rp, col, torch.ones(3), 4)
+
+
+def test_error_paths_are_loud(gpu_ctx_factory):
+    """Invalid calls return error codes with a message (raised as EpetError),
+    never a silent result: too many frames, a null image, a negative chord
+    capacity, tuning without the transposed copy."""
+    import ctypes
+    from epetreco_amd._lib import EpetError, lib
+    from epetreco_amd.api import Matrix
+    ctx = gpu_ctx_factory("small", seed=0)
+    rng = np.random.default_rng(2)
+    rp, col, val = _random_csr(rng, 200, 300, 20)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), 300)
+    y = torch.ones(200, 65, device="cuda")
+    s = torch.ones(300, device="cuda")
+    X = torch.full((300, 65), 1.0 / 300, device="cuda")
+    with pytest.raises(EpetError, match="frames"):
+        ctx.mlem(A, y, s, X, 1)
+    y1 = _t(np.ones(200), torch.float32)
+    rc = lib().epet_mlem(ctx.handle, A.handle, 1, y1.data_ptr(), s.data_ptr(), None, 1, 0, 0.0,
+                         None, None)
+    assert rc != 0 and "null" in lib().epet_last_error().decode()
+    with pytest.raises(EpetError, match="negative"):
+        ctx.getChords(torch.zeros(1, 6, device="cuda"), -1)
+    with pytest.raises(EpetError, match="transpose"):
+        ctx.mlem_tune(A, 8)
