@@ -42,12 +42,29 @@ def event_time_ms(fn, reps):
     return st.elapsed_time(en) / reps
 
 
-def cpu_baseline(A, y_dev, s_dev, target_s=15.0):
-    """The oracle's FP32 MLEM iteration (OpenMP) on a leading row sample of A,
-    scaled to the full row count by nonzeros."""
+def host_info():
+    """Host cores and CPU model of the machine the CPU legs run on."""
+    from oracle import oracle as O
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": aff, "threads": int(O.lib().eo_max_threads()),
+            "cpu_model": model}
+
+
+def cpu_baseline(A, y_dev, s_dev, host, target_s=15.0):
+    """The oracle's FP32 MLEM iteration (OpenMP over every thread the process
+    may use) on a leading row sample of A, scaled to the full row count by
+    nonzeros."""
     import ctypes
     import numpy as np
-    import torch
     from oracle import oracle as O
     rp, col, val, _ = A.to_torch()
     rp_h = rp.cpu().numpy()
@@ -58,6 +75,7 @@ def cpu_baseline(A, y_dev, s_dev, target_s=15.0):
     nnz_s = int(rp_h[msamp])
     colh = col[:nnz_s].cpu().numpy()
     valh = val[:nnz_s].cpu().numpy()
+    del rp, col, val
     rph = rp_h[:msamp + 1].copy()
     yh = y_dev[:msamp].cpu().numpy()
     sh = s_dev.cpu().numpy()
@@ -66,7 +84,7 @@ def cpu_baseline(A, y_dev, s_dev, target_s=15.0):
     yhat = np.zeros(msamp, dtype=np.float32)
     c = np.zeros(A.n, dtype=np.float32)
     L = O.lib()
-    nth = min(16, os.cpu_count() or 1)
+    nth = host["threads"]
     args = (msamp, A.n, O.ptr(valh, ctypes.c_float), O.ptr(rph, ctypes.c_int64),
             O.ptr(colh, ctypes.c_int), O.ptr(yh, ctypes.c_float), O.ptr(sh, ctypes.c_float),
             O.ptr(x, ctypes.c_float), O.ptr(yhat, ctypes.c_float), O.ptr(c, ctypes.c_float), nth)
@@ -79,9 +97,29 @@ def cpu_baseline(A, y_dev, s_dev, target_s=15.0):
         times.append(time.perf_counter() - t0)
     t_it = float(np.median(times)) * (A.nnz / nnz_s)
     return {"value": 1.0 / t_it, "unit": "frame-iter/s", "cores": nth, "kind": "port",
+            "host": host,
             "sample": f"oracle FP32 MLEM iteration (OpenMP, {nth} threads) on the first "
                       f"{msamp} of {A.m} measured rows ({nnz_s} of {A.nnz} nonzeros), "
                       f"median of {len(times)}, scaled by nonzeros"}
+
+
+def cpu_generation(setup, grid, nrays, host, frac=5e-4, seed=0):
+    """The oracle's system-matrix generation (the reference's full candidate
+    scan plus the Monte-Carlo slab values of every candidate) on a seeded
+    channel sample, extrapolated linearly to every channel."""
+    import numpy as np
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed + 99)
+    ns = max(1, int(round(frac * setup.nchannels)))
+    lst = np.sort(rng.choice(setup.nchannels, ns, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    rp, _, _ = O.sm_rows(setup, grid, lst, nrays, 0)
+    t = time.perf_counter() - t0
+    return {"seconds_all_channels": t * setup.nchannels / ns, "sample_channels": ns,
+            "sample_seconds": t, "sample_nonzeros": int(rp[-1]), "threads": host["threads"],
+            "sample": f"{ns} seeded channels of {setup.nchannels} ({100.0 * ns / setup.nchannels:.3g}%), "
+                      f"oracle full voxel scan + {nrays}-ray values, OpenMP {host['threads']} "
+                      f"threads, extrapolated linearly"}
 
 
 def spmv_bytes(nnz, m, V):
@@ -115,6 +153,11 @@ def main():
     ap.add_argument("--workload", default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--accuracy-iters", type=int, default=20,
+                    help="batched MLEM iterations of the accuracy check against the FP64 oracle "
+                         "(0: skip)")
+    ap.add_argument("--cpu-generation", type=int, default=1,
+                    help="time the oracle's generation on a channel sample (0: skip)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="after building the problem, launch only the forward SpMV, the back "
                          "projection and a 1 GiB device copy (3 times each) for counter runs")
@@ -129,7 +172,7 @@ def main():
     import torch
     from epetreco_amd import geometry as G
     from epetreco_amd import phantom as PH
-    from epetreco_amd.api import Context, OP_N, OP_T, VAL_BF16
+    from epetreco_amd.api import Context, OP_N, OP_T, VAL_BF16, VAL_F16
 
     setup, grid, nrays = G.workload(args.workload)
     ctx = Context(0)
@@ -212,8 +255,9 @@ def main():
         f"= {bwd_gbps:.0f} GB/s")
 
     # ---- batched multi-frame MLEM, 32 frames: BF16 storage (headline) and
-    # FP32 storage, both through the transposed copy and the autotuned kernels
+    # FP32 storage, through the transposed copy
     batched = None
+    host = host_info()
     if not args.no_batched:
         F = args.frames
         imgs, totals = PH.dynamic_series(grid, args.workload, F, seed=0)
@@ -224,18 +268,17 @@ def main():
         gen = torch.Generator(device="cuda").manual_seed(1)
         Yc = torch.poisson(Yb, generator=gen).contiguous()
         del Yb, X
+        # accuracy frames: the fewest and the most counts, the first and the last
+        pick = sorted({int(np.argmin(totals)), int(np.argmax(totals)), 0, F - 1})
+        acc_iters = args.accuracy_iters
 
         def run_batched(M, label):
             t0 = time.perf_counter()
             M.attach_transpose()
             torch.cuda.synchronize()
             t_tr = time.perf_counter() - t0
-            cfg, tms = ctx.mlem_tune(M, F)
-            say(f"  {label}: transpose {t_tr:.2f} s; gather-kernel fallback tuned: forward R={cfg[0]} "
-                f"P={cfg[1]} ({tms[0]:.3f} ms), back projection R={cfg[2]} P={cfg[3]} "
-                f"({tms[1]:.3f} ms); the iteration runs the LDS-tiled multi-frame kernels")
             Xb = torch.full((V, F), 1.0 / V, device="cuda")
-            ctx.mlem(M, Yc, prob.s, Xb, 1, use_graph=True)
+            ctx.mlem(M, Yc, prob.s, Xb, 1, use_graph=True)       # one-off copies, warm-up
             Xb.fill_(1.0 / V)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -244,34 +287,86 @@ def main():
             t_b = time.perf_counter() - t0
             fis = args.batched_steps * F / t_b
             say(f"  {label} x{F}: {1e3 * t_b / args.batched_steps:.2f} ms/iteration, "
-                f"{fis:.1f} frame-iter/s ({fis / plain_fis:.2f}x plain FP32)")
-            return Xb, {"frame_iter_per_s": fis, "ms_per_iteration": 1e3 * t_b / args.batched_steps,
-                        "speedup_vs_plain_fp32": fis / plain_fis,
-                        "kernels": "LDS-tiled multi-frame (k_mfl, 8 frames per pass)",
-                        "gather_fallback_tuned": {"fwd_rows_per_wave": cfg[0], "fwd_frame_tile": cfg[1],
-                                  "bwd_rows_per_wave": cfg[2], "bwd_frame_tile": cfg[3],
-                                  "fwd_ms": tms[0], "bwd_ms": tms[1]}}
+                f"{fis:.1f} frame-iter/s ({fis / plain_fis:.2f}x plain FP32); transpose {t_tr:.2f} s")
+            res = {"frame_iter_per_s": fis, "ms_per_iteration": 1e3 * t_b / args.batched_steps,
+                   "speedup_vs_plain_fp32": fis / plain_fis, "iterations_timed": args.batched_steps,
+                   "kernels": ctx.batched_config(M, F)}
+            Xa = None
+            if acc_iters > 0:
+                Xa = torch.full((V, len(pick)), 1.0 / V, device="cuda")
+                ctx.mlem(M, Yc[:, pick].contiguous(), prob.s, Xa, acc_iters, use_graph=True)
+                Xa = Xa.cpu().numpy()
+            return Xb, res, Xa
 
         say(f"batched MLEM, {F} frames")
-        Ab = A.convert(VAL_BF16)
-        Xb, rb = run_batched(Ab, "BF16 storage")
-        Ab.close()
-        del Ab
-        gc.collect()
-        Xf, rf = run_batched(A, "FP32 storage")
-        err = ((Xb - Xf).norm(dim=0) / Xf.norm(dim=0)).cpu().numpy()
-        say(f"  max frame rel-L2, BF16 vs FP32 storage after {args.batched_steps} iterations: "
-            f"{err.max():.2e}")
-        batched = {"frames": F, "bf16_row_scale": rb, "fp32": rf,
-                   "max_frame_rel_l2_bf16_vs_fp32_storage": float(err.max())}
-        del Xb, Xf
+        runs = {}
+        for key, vt, label in (("bf16_row_scale", VAL_BF16, "BF16 storage"),
+                               ("fp16_row_scale", VAL_F16, "FP16 storage")):
+            M = A.convert(vt)
+            runs[key] = run_batched(M, label)
+            M.close()
+            del M
+            gc.collect()
+            torch.cuda.empty_cache()
+        runs["fp32"] = run_batched(A, "FP32 storage")
+        Xf = runs["fp32"][0]
+        batched = {"frames": F}
+        for key, (Xk, rk, _) in runs.items():
+            batched[key] = rk
+            if key != "fp32":
+                err = ((Xk - Xf).norm(dim=0) / Xf.norm(dim=0)).cpu().numpy()
+                rk["max_frame_rel_l2_vs_fp32_storage"] = float(err.max())
+                say(f"  max frame rel-L2, {key} vs FP32 storage after {args.batched_steps} "
+                    f"iterations: {err.max():.2e}")
+        rb, rh, rf = runs["bf16_row_scale"][1], runs["fp16_row_scale"][1], runs["fp32"][1]
+        Ab_acc, Ah_acc, Af_acc = (runs[k][2] for k in ("bf16_row_scale", "fp16_row_scale", "fp32"))
+        del runs, Xf
+        if acc_iters > 0:
+            # FP64 oracle MLEM on the FP32 matrix for the picked frames
+            from oracle import oracle as O
+            say(f"  FP64 oracle: frames {pick}, {acc_iters} iterations, {host['threads']} threads")
+            t0 = time.perf_counter()
+            rp, col, val, _ = A.to_torch()
+            rp, col, val = rp.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+            torch.cuda.empty_cache()
+            X64 = O.mlem_f64_frames_par(A.m, V, val, rp, col, Yc[:, pick].cpu().numpy(),
+                                        prob.s.cpu().numpy(), np.full((V, len(pick)), 1.0 / V),
+                                        acc_iters)
+            del rp, col, val
+            t_o = time.perf_counter() - t0
+
+            def errs(Xg):
+                return [float(np.linalg.norm(Xg[:, k].astype(np.float64) - X64[:, k]) /
+                              np.linalg.norm(X64[:, k])) for k in range(len(pick))]
+
+            acc = {"frames": pick, "frame_counts": [float(totals[k]) for k in pick],
+                   "iterations": acc_iters, "bar": 1e-3,
+                   "reference": f"FP64 oracle MLEM on the FP32 matrix (OpenMP, {host['threads']} threads)"}
+            for name, rk, Xa in (("BF16", rb, Ab_acc), ("FP16", rh, Ah_acc), ("FP32", rf, Af_acc)):
+                ek = errs(Xa)
+                rk["max_frame_rel_l2_vs_fp64"] = max(ek)
+                rk["frame_rel_l2_vs_fp64"] = ek
+                rk["meets_1e-3_vs_fp64"] = bool(max(ek) <= 1e-3)
+                say(f"  rel-L2 vs FP64 after {acc_iters} iterations, frames {pick}, {name}: "
+                    + " ".join(f"{e:.2e}" for e in ek))
+            say(f"  (oracle {t_o:.1f} s)")
+            batched["accuracy"] = acc
 
     cpu = None
+    gen_cpu = None
     if not args.no_cpu:
         say("CPU baseline (oracle) on a row sample ...")
-        cpu = cpu_baseline(A, prob.y, prob.s)
-        say(f"  {cpu['value']:.3f} frame-iter/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(A, prob.y, prob.s, host)
+        say(f"  {cpu['value']:.3f} frame-iter/s on {cpu['cores']} threads ({host['cpu_model']})")
+        if args.cpu_generation:
+            say("CPU generation (oracle) on a channel sample ...")
+            gen_cpu = cpu_generation(setup, grid, nrays, host)
+            say(f"  {gen_cpu['sample_channels']} channels in {gen_cpu['sample_seconds']:.1f} s; "
+                f"all channels ~{gen_cpu['seconds_all_channels']:.0f} s")
+            cpu["generation"] = gen_cpu
 
+    algo_frac = fwd_gbps / HBM_PEAK_GBPS
+    traffic = measured_traffic(args.workload, "k_rows_lds<ValF32,4,16,0>")
     out = {
         "metric": "MLEM frame-iter/s + SpMV HBM GB/s at 52^3/5.1M ch; batched BF16 vs plain FP32",
         "value": plain_fis,
@@ -295,8 +390,8 @@ def main():
         "transpose_s": tr_s,
         "batched": batched,
         "roofline": {"bound": "hbm", "achieved": fwd_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": fwd_gbps / HBM_PEAK_GBPS,
-                     "traffic": measured_traffic(args.workload, "k_rows_lds<ValF32,4,16,0>"),
+                     "frac": algo_frac, "traffic": traffic,
+                     "frac_physical": (traffic / fwd_ms / 1e6 / HBM_PEAK_GBPS) if traffic else None,
                      "kernel": "forward SpMV (k_rows_lds<ValF32,4,16,0> + k_lds_reduce, timed together)",
                      "algorithmic_bytes": B_fwd, "avg_ms": fwd_ms},
         "cpu_baseline": cpu,

@@ -80,6 +80,7 @@ SIGNATURES = {
                                  ctypes.c_double, c_double_p, c_int_p]),
     "epet_getChords": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, P, P, P]),
     "epet_mlem_tune": (ctypes.c_int, [P, P, ctypes.c_int, c_int_p, c_float_p]),
+    "epet_mlem_kernels": (ctypes.c_int, [P, P, ctypes.c_int, c_int_p, c_float_p]),
     "epet_nInChunk": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
     "epet_chunkPtr": (ctypes.c_uint32, [ctypes.c_uint32] * 2),
     "epet_nChunks": (ctypes.c_uint32, [ctypes.c_ulonglong] * 2),

@@ -21,6 +21,8 @@ import torch
 
 from . import _lib
 from ._lib import OP_N, OP_T, VAL_BF16, VAL_F16, VAL_F32, EpetError, check, lib
+
+EPET_ERR_ARG = 1
 from .geometry import MeasurementSetup, VoxelGrid
 
 __all__ = ["Context", "Matrix", "EpetError", "OP_N", "OP_T", "VAL_F32", "VAL_BF16", "VAL_F16"]
@@ -40,6 +42,22 @@ def _p(t: Optional[torch.Tensor]):
 def _check_dtype(t: torch.Tensor, dtype, name: str):
     if t.dtype != dtype:
         raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+
+
+def _check_vec(t, n: int, name: str, device) -> None:
+    """A float32 device tensor of exactly n elements on `device` (checked on the
+    host before any device call: a wrong size would make a kernel read or write
+    past the allocation)."""
+    if not isinstance(t, torch.Tensor):
+        raise EpetError(EPET_ERR_ARG, f"{name}: expected a torch tensor")
+    if t.dtype != torch.float32:
+        raise EpetError(EPET_ERR_ARG, f"{name} must be float32, got {t.dtype}")
+    if t.device != device:
+        raise EpetError(EPET_ERR_ARG, f"{name} is on {t.device}, the context is on {device}")
+    if not t.is_contiguous():
+        raise EpetError(EPET_ERR_ARG, f"{name} must be contiguous")
+    if t.numel() != n:
+        raise EpetError(EPET_ERR_ARG, f"{name} has {t.numel()} elements, expected {n}")
 
 
 class Context:
@@ -241,10 +259,44 @@ class Context:
                                    ms.ctypes.data_as(_lib.c_float_p)))
         return tuple(int(v) for v in cfg), tuple(float(v) for v in ms)
 
+    def batched_config(self, A: "Matrix", frames: int) -> dict:
+        """The multi-frame path and launch shape epet_mlem runs for `frames`
+        frames on A (as tuned by its first multi-frame call)."""
+        info = np.zeros(10, dtype=np.int32)
+        ms = np.zeros(2, dtype=np.float32)
+        check(lib().epet_mlem_kernels(self._h, A.handle, int(frames),
+                                      info.ctypes.data_as(_lib.c_int_p),
+                                      ms.ctypes.data_as(_lib.c_float_p)))
+        path = {2: "blocked", 1: "gather", 0: "untuned"}[int(info[0])]
+        if path == "blocked":
+            return {"path": "blocked (register-accumulated, LDS-staged tiles, k_blk)",
+                    "forward": {"frames_per_lane": int(info[1]), "row_groups_per_lane": int(info[2]),
+                                "rows_per_workgroup": int(info[3]), "splits": int(info[4]),
+                                "tuned_ms": float(ms[0])},
+                    "back_projection": {"frames_per_lane": int(info[5]),
+                                        "row_groups_per_lane": int(info[6]),
+                                        "rows_per_workgroup": int(info[7]), "splits": int(info[8]),
+                                        "tuned_ms": float(ms[1])}}
+        if path == "gather":
+            return {"path": "gather kernels (k_mf_fwd / k_mf_bwd)",
+                    "forward": {"frame_tile": int(info[1]), "rows_per_wavefront": int(info[2]),
+                                "tuned_ms": float(ms[0])},
+                    "back_projection": {"frame_tile": int(info[5]),
+                                        "rows_per_wavefront": int(info[6]),
+                                        "tuned_ms": float(ms[1])}}
+        return {"path": path}
+
     def mlem(self, A: "Matrix", y: torch.Tensor, s: torch.Tensor, x: torch.Tensor,
              niter: int, use_graph: bool = False, tol: float = 0.0):
         """In-place MLEM on x (n x F, frame-minor); returns (loglik[niter, F], iters[F])."""
         F = 1 if x.dim() == 1 else x.shape[1]
+        if not 1 <= F <= 64:
+            raise EpetError(EPET_ERR_ARG, f"frames must be in [1, 64], got {F}")
+        if y.dim() != x.dim() or (x.dim() == 2 and y.shape[1] != F):
+            raise EpetError(EPET_ERR_ARG, f"y {tuple(y.shape)} and x {tuple(x.shape)} disagree on the frames")
+        _check_vec(y, A.m * F, "y", self.device)
+        _check_vec(s, A.n, "s", self.device)
+        _check_vec(x, A.n * F, "x", self.device)
         ll = np.zeros((max(niter, 1), F), dtype=np.float64)
         iters = np.zeros(F, dtype=np.int32)
         check(lib().epet_mlem(self._h, A.handle, F, _p(y), _p(s), _p(x), niter, int(use_graph),
@@ -319,19 +371,33 @@ class Matrix:
         return trp, tc[:self.nnz], tv[:self.nnz], ts
 
     def spmv(self, op: int, x: torch.Tensor, y: Optional[torch.Tensor] = None,
-             alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+             alpha: float = 1.0, beta: float = 0.0, ctx: Optional[Context] = None) -> torch.Tensor:
+        """y = alpha op(A) x + beta y (on `ctx`, the matrix's own context by default)."""
+        ctx = ctx or self.ctx
+        if op not in (OP_N, OP_T):
+            raise EpetError(EPET_ERR_ARG, f"unknown operation {op}")
+        nin, nout = (self.n, self.m) if op == OP_N else (self.m, self.n)
         if y is None:
-            y = torch.zeros(self.m if op == OP_N else self.n, dtype=torch.float32,
-                            device=self.ctx.device)
-        check(lib().epet_matrix_spmv(self.ctx.handle, self._h, op, alpha, _p(x), beta, _p(y)))
+            y = torch.zeros(nout, dtype=torch.float32, device=self.ctx.device)
+        _check_vec(x, nin, "x", self.ctx.device)
+        _check_vec(y, nout, "y", self.ctx.device)
+        check(lib().epet_matrix_spmv(ctx.handle, self._h, op, alpha, _p(x), beta, _p(y)))
         return y
 
     def spmm(self, op: int, X: torch.Tensor, Y: Optional[torch.Tensor] = None,
              alpha: float = 1.0, beta: float = 0.0) -> torch.Tensor:
+        if op not in (OP_N, OP_T):
+            raise EpetError(EPET_ERR_ARG, f"unknown operation {op}")
+        if X.dim() != 2:
+            raise EpetError(EPET_ERR_ARG, "X must be 2-D (rows x frames, frame-minor)")
         F = X.shape[1]
+        nin, nout = (self.n, self.m) if op == OP_N else (self.m, self.n)
         if Y is None:
-            Y = torch.zeros((self.m if op == OP_N else self.n, F), dtype=torch.float32,
-                            device=self.ctx.device)
+            Y = torch.zeros((nout, F), dtype=torch.float32, device=self.ctx.device)
+        if Y.dim() != 2 or Y.shape[1] != F:
+            raise EpetError(EPET_ERR_ARG, f"Y {tuple(Y.shape)} must have {F} frames")
+        _check_vec(X, nin * F, "X", self.ctx.device)
+        _check_vec(Y, nout * F, "Y", self.ctx.device)
         check(lib().epet_matrix_spmm(self.ctx.handle, self._h, op, F, alpha, _p(X), beta, _p(Y)))
         return Y
 

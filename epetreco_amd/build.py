@@ -33,7 +33,7 @@ SOURCES = {
     "mlem.hip": [],
     "mframe.hip": [],
     "lds.hip": [],
-    "mlds.hip": [],
+    "blk.hip": [],
 }
 
 

@@ -1,10 +1,12 @@
 // core.hip -- errors, contexts, geometry upload, device memory, prefix sums,
 // chunk helpers of libepet.so.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
 #include "epet_block.hpp"
+
 #include "epet_internal.hpp"
 
 #ifndef M_PI
@@ -213,6 +215,18 @@ int epet_ctx_create(epet_ctx** out, int device, void* stream) {
   EPET_HIP(hipSetDevice(device));
   epet_ctx* c = new epet_ctx();
   c->device = device;
+  {
+    const char* e = getenv("EPET_LDS");
+    c->opt.lds = !(e && e[0] == '0');
+    e = getenv("EPET_LDS_MAP");
+    c->opt.lds_map = e && e[0] == '1';
+    e = getenv("EPET_BP");
+    const std::string bp = e ? e : "";
+    c->opt.bp = bp == "atomic" ? BP_ATOMIC : bp == "tiled" ? BP_TILED : bp == "rows" ? BP_ROWS
+                                                                                : BP_DEFAULT;
+    e = getenv("EPET_BATCHED");
+    c->opt.batched_gather = e && std::string(e) == "gather";
+  }
   c->stream = (hipStream_t)stream;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;

@@ -58,7 +58,22 @@ struct Geo {
 }  // namespace epet
 
 // ---------------------------------------------------------------- context
+// Path choices of a context, read once from the environment when it is created
+// (for comparisons and measurements; the defaults are the fast paths):
+//   EPET_LDS=0         single-frame products without LDS-tiled copies
+//   EPET_LDS_MAP=1     LDS-tiled copies of grid matrices in y-major voxel order
+//   EPET_BP=atomic|tiled|rows   single-frame back-projection variant
+//   EPET_BATCHED=gather         multi-frame MLEM with the gather kernels
+enum { BP_DEFAULT = 0, BP_ATOMIC = 1, BP_TILED = 2, BP_ROWS = 3 };
+struct EpetOpts {
+  bool lds = true;
+  bool lds_map = false;
+  int bp = BP_DEFAULT;
+  bool batched_gather = false;
+};
+
 struct epet_ctx {
+  EpetOpts opt;
   int device = 0;
   hipStream_t stream = nullptr;
   bool have_geo = false;
@@ -94,6 +109,16 @@ __host__ __device__ inline long long colmap_inv(long long pc, const ColMap& mp) 
   return (ix * mp.ny + iy) * mp.nz + iz;
 }
 
+// Geometry a generated matrix was built on: the channel-id radices of its
+// rows (MeasurementSetupLinIndex.hpp:40-44) and the voxel grid of its columns
+// (z fastest).  All zero when unknown (caller-built matrices).
+struct MatGeo {
+  int na = 0, n0z = 0, n0y = 0, n1z = 0, n1y = 0;
+  int nx = 0, ny = 0, nz = 0;
+};
+
+namespace epet { struct BlkCopy; }
+
 struct epet_matrix {
   int m = 0, n = 0;
   long long nnz = 0;
@@ -102,6 +127,8 @@ struct epet_matrix {
   int* col = nullptr;          // nnz
   void* val = nullptr;         // nnz x (4 | 2) bytes
   float* rowscale = nullptr;   // m (reduced-precision storage only)
+  int* row_ids = nullptr;      // m: channel id of every row, when known
+  MatGeo mg;
   int device = 0;
   bool owns = true;
   // tuning chosen for this matrix (lanes per row of the row-parallel kernels)
@@ -135,16 +162,9 @@ struct epet_matrix {
   float* lds_partial = nullptr;
   ColMap lds_map;
   bool lds_failed = false;
-  // multi-frame LDS products: tile-major copy for tiles of LDS_TILE/mfl_P
-  // columns (P frames of each column in LDS), and partials (mfl_T x m x F)
-  int mfl_P = 0, mfl_T = 0, mfl_tile = 0;   // tile: LDS floats per workgroup
-  int64_t* mfl_off = nullptr;
-  void* mfl_val = nullptr;
-  uint16_t* mfl_col16 = nullptr;
-  float* mfl_partial = nullptr;
-  long long mfl_partial_len = 0;
-  ColMap mfl_map;
-  bool mfl_failed = false;
+  // blocked multi-frame copy (register-accumulated products, one per padded
+  // frame count in use) and its autotuned launch shape
+  epet::BlkCopy* blk = nullptr;
   // multi-frame kernel configuration picked by the autotuner for mf_F frames
   // (rows per wavefront R, frames per lane tile P) and the times it measured
   int mf_F = 0, mf_fwd_R = 1, mf_fwd_P = 32, mf_bwd_R = 1, mf_bwd_P = 32;
@@ -161,6 +181,8 @@ constexpr int LDS_TILE = 32768;
 
 
 namespace epet {
+
+void blk_release(epet_matrix* A);   // frees the blocked copies of A
 
 // device scratch owned by the context, grown on demand (stream-ordered use)
 int ctx_scratch(epet_ctx* ctx, size_t bytes, void** out);

@@ -271,10 +271,8 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
   }
   // each workgroup owns its CU's LDS; several workgroups per CU in all, so
   // that tiles and row ranges of unequal weight even out: 8 for the forward
-  // and 16 for the back projection were fastest at 52^3 (EPET_LDS_WG
-  // overrides them, for measurements)
-  const char* ew = getenv("EPET_LDS_WG");
-  const long long per_cu = ew ? std::max(1, atoi(ew)) : (BWD ? 16 : 8);
+  // and 16 for the back projection were fastest at 52^3
+  const long long per_cu = BWD ? 16 : 8;
   const long long G = std::max<long long>(1, (per_cu * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
@@ -284,22 +282,12 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
                                             A->lds_partial, A->lds_map);
 }
 
-// Rows per wavefront from the mean segment length (EPET_LDS_R = 1, 2, 4, 8 or
-// 16 forces it, for measurements; R = 8 takes 24 entries per lane per pass).
+// Rows per wavefront from the mean segment length.
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
-  int R = seg >= 384.0 ? 2 : seg >= 48.0 ? 4 : 16;
-  const char* er = getenv("EPET_LDS_R");
-  if (er) R = atoi(er);
-  if (R == 1) launch_rows_lds<VT, 1, 16, BWD>(ctx, A, x);
-  else if (R == 2) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
-  else if (R == 4) {
-    const char* ej = getenv("EPET_LDS_J");   // 12: 192 entries per row and pass
-    if (ej && atoi(ej) == 12) launch_rows_lds<VT, 4, 12, BWD>(ctx, A, x);
-    else launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
-  }
-  else if (R == 8) launch_rows_lds<VT, 8, 24, BWD>(ctx, A, x);
+  if (seg >= 384.0) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
+  else if (seg >= 48.0) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
   else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
 }
 
@@ -312,12 +300,10 @@ static void rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   }
 }
 
-// Whether the LDS-tiled products apply (EPET_LDS=0 turns them off for
-// measurements): long rows, ascending columns.
-static bool lds_wanted(const epet_matrix* A) {
-  const char* e = getenv("EPET_LDS");
-  if (e && e[0] == '0') return false;
-  return A->m > 0 && A->cols_strict && (double)A->nnz / A->m >= 64.0;
+// Whether the LDS-tiled products apply (the context may turn them off): long
+// rows, ascending columns.
+static bool lds_wanted(const epet_ctx* ctx, const epet_matrix* A) {
+  return ctx->opt.lds && A->m > 0 && A->cols_strict && (double)A->nnz / A->m >= 64.0;
 }
 
 static void lds_release(epet_matrix* A) {
@@ -425,22 +411,14 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, ColMap mp, int
   return EPET_OK;
 }
 
-int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
-
-// Builds the tile-major copy and the partial buffer once (allocates and
-// synchronises: call outside graph capture).  voxel_columns: the columns are
-// image voxels (a forward projector), so they may be renumbered y-major.  Leaves the matrix without them
-// when an allocation fails; the other products then run.
-// y-major column order when the columns are the voxels of the context's grid
-// and EPET_LDS_MAP=1.  Off by default: rows then meet fewer tiles, but their
-// segments become very uneven (most empty, a few long), which leaves the row
-// groups of a wavefront idle; at 52^3 the forward product took 5.6 ms instead
-// of 4.5 ms and the 32-frame iteration 119 ms instead of 94 ms.
-ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A) {
+// y-major column order for the copy of a grid matrix when the context asks
+// for it (EPET_LDS_MAP=1).  Off by default: rows then meet fewer tiles, but
+// their segments become very uneven (most empty, a few long), which leaves the
+// row groups of a wavefront idle; at 52^3 the forward product took 5.6 ms
+// instead of 4.5 ms.
+static ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A) {
   ColMap mp;
-  const char* e = getenv("EPET_LDS_MAP");
-  if (!(e && e[0] == '1')) return mp;
-  if (!ctx->have_geo) return mp;
+  if (!ctx->opt.lds_map || !ctx->have_geo) return mp;
   const int* gn = ctx->geo.gn;
   if ((long long)gn[0] * gn[1] * gn[2] != A->n || gn[1] > 4096) return mp;
   mp.nx = gn[0];
@@ -449,12 +427,12 @@ ColMap voxel_map(const epet_ctx* ctx, const epet_matrix* A) {
   return mp;
 }
 
-int lds_prepare(epet_ctx* ctx, epet_matrix* A) {
-  return lds_prepare_map(ctx, A, false);
-}
-
+// Builds the tile-major copy and the partial buffer once (allocates and
+// synchronises: call outside graph capture).  voxel_columns: the columns are
+// image voxels (a forward projector), so they may be renumbered.  Leaves the
+// matrix without them when an allocation fails; the other products then run.
 int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns) {
-  if (A->lds_off || A->lds_failed || !lds_wanted(A)) return EPET_OK;
+  if (A->lds_off || A->lds_failed || !lds_wanted(ctx, A)) return EPET_OK;
   int T = 0;
   const ColMap mp = voxel_columns ? voxel_map(ctx, A) : ColMap{};
   EPET_TRY(build_tile_major(ctx, A, LDS_TILE, mp, &T, &A->lds_off, &A->lds_val, &A->lds_col16));
@@ -470,7 +448,11 @@ int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns) {
   return EPET_OK;
 }
 
-bool lds_ready(const epet_matrix* A) { return A->lds_off != nullptr && lds_wanted(A); }
+int lds_prepare(epet_ctx* ctx, epet_matrix* A) { return lds_prepare_map(ctx, A, false); }
+
+bool lds_ready(const epet_ctx* ctx, const epet_matrix* A) {
+  return A->lds_off != nullptr && lds_wanted(ctx, A);
+}
 
 // y = alpha * diag(scale) A x + beta y (scale: the row scale when use_scale;
 // bwd: A is a transposed copy)

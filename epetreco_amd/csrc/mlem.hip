@@ -29,19 +29,28 @@ int matrix_spmm(epet_ctx* ctx, const epet_matrix* A, int op, int F, float alpha,
                 float beta, float* Y);
 int pick_lanes(long long nnz, int m);
 int bp_prepare(epet_ctx* ctx, epet_matrix* A);
-bool bp_use_transpose(const epet_matrix* A);
+bool bp_use_transpose(const epet_ctx* ctx, const epet_matrix* A);
 int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
                         float beta, float* y);
 int lds_prepare(epet_ctx* ctx, epet_matrix* A);
 int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
-bool lds_ready(const epet_matrix* A);
+bool lds_ready(const epet_ctx* ctx, const epet_matrix* A);
 int lds_fwd_ratio(epet_ctx* ctx, const epet_matrix* A, const float* x, const float* y, float* e,
                   int fold, const int* itp, double* loglik);
 int mf_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
 int mf_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
-int mfl_prepare(epet_ctx* ctx, epet_matrix* A, int F);
-bool mfl_ready(const epet_matrix* A, int F);
-int mfl_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
+int blk_prepare(epet_ctx* ctx, epet_matrix* A, int F);
+bool blk_ready(const epet_matrix* A, int F);
+bool blk_built(const epet_matrix* A, int F);
+int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps);
+int blk_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a);
+int blk_info(const epet_matrix* A, int F, int info[9], float ms[2]);
+
+// whether multi-frame MLEM on A runs the blocked products (copies built and
+// tuned for F frames, not turned off by the context)
+static bool use_blk(const epet_ctx* ctx, const epet_matrix* A, int F) {
+  return !ctx->opt.batched_gather && blk_ready(A, F);
+}
 
 // ------------------------------------------------------------------ element-wise
 __global__ __launch_bounds__(256) void k_divides(float* __restrict__ res, const float* __restrict__ a,
@@ -271,6 +280,10 @@ __global__ void k_advance(int* itp) {
   if (threadIdx.x == 0) *itp += 1;
 }
 
+__global__ void k_fill_i32_one(int* p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
 __global__ __launch_bounds__(256) void k_fill_f32(float* __restrict__ p, long long n, float v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -335,13 +348,13 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
   int rc;
   if (F > 1 && A->tr) {
     const MfArgs a{F, x, y, e, c, itp, loglik};
-    rc = mfl_ready(A, F) ? mfl_iteration(ctx, A, a) : mf_iteration(ctx, A, a);
+    rc = use_blk(ctx, A, F) ? blk_iteration(ctx, A, a) : mf_iteration(ctx, A, a);
   } else {
     // single frame through the transposed copy: the ratio carries the row scale
     // and the back projection is a row-streaming product over the transpose
-    const bool via_tr = F == 1 && bp_use_transpose(A);
+    const bool via_tr = F == 1 && bp_use_transpose(ctx, A);
     const int fold = via_tr && A->rowscale ? 1 : 0;
-    if (F == 1 && lds_ready(A)) rc = lds_fwd_ratio(ctx, A, x, y, e, fold, itp, loglik);
+    if (F == 1 && lds_ready(ctx, A)) rc = lds_fwd_ratio(ctx, A, x, y, e, fold, itp, loglik);
     else switch (A->value_type) {
       case EPET_VAL_F32: rc = launch_fwd_ratio<ValF32>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
       case EPET_VAL_BF16: rc = launch_fwd_ratio<ValBF16>(st, ctx->num_cus, A, F, x, y, e, fold, itp, loglik); break;
@@ -467,19 +480,25 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   // back-projection plan (allocates) and multi-frame tuning before any capture
   if (F == 1) {
     rc = lds_prepare_map(ctx, const_cast<epet_matrix*>(A), true);
-    if (rc == EPET_OK && bp_use_transpose(A)) rc = lds_prepare(ctx, A->tr);
-    if (rc == EPET_OK && !bp_use_transpose(A)) rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
+    if (rc == EPET_OK && bp_use_transpose(ctx, A)) rc = lds_prepare(ctx, A->tr);
+    if (rc == EPET_OK && !bp_use_transpose(ctx, A)) rc = bp_prepare(ctx, const_cast<epet_matrix*>(A));
     if (rc != EPET_OK) { cleanup(); return rc; }
-  } else if (F > 1 && A->tr) {
-    rc = mfl_prepare(ctx, const_cast<epet_matrix*>(A), F);
+  } else if (F > 1 && A->tr && !ctx->opt.batched_gather) {
+    // blocked copies (built once per padded frame count) and their launch shape
+    rc = blk_prepare(ctx, const_cast<epet_matrix*>(A), F);
+    // copies that could not be built leave the gather kernels in place
+    if (rc == EPET_OK && blk_built(A, F) && !blk_ready(A, F)) {
+      const MfArgs a{F, x, y, e, c, itp, nullptr};
+      rc = blk_tune(ctx, const_cast<epet_matrix*>(A), a, 2);
+    }
     if (rc != EPET_OK) { cleanup(); return rc; }
   }
-  if (F > 1 && A->tr && !mfl_ready(A, F) && A->mf_F != F) {
+  if (F > 1 && A->tr && !use_blk(ctx, A, F) && A->mf_F != F) {
     const MfArgs a{F, x, y, e, c, itp, nullptr};
     rc = mf_tune(ctx, const_cast<epet_matrix*>(A), a, 2);
     if (rc != EPET_OK) { cleanup(); return rc; }
-    k_fill_f32<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(c, nv, 0.0f);
   }
+  k_fill_f32<<<grid_for(nv, 256, 1ll << 31), 256, 0, st>>>(c, nv, 0.0f);
   const bool stopping = tol > 0.0 && frame_iters != nullptr;
   std::vector<int> act(F, 1), iters(F, 0);
   std::vector<double> hll((size_t)std::max(1, niter) * F, 0.0);
@@ -553,25 +572,73 @@ int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms) {
   float *x = nullptr, *y = nullptr, *e = nullptr, *c = nullptr;
   int* itp = nullptr;
   const long long nx = (long long)A->n * F, ny = (long long)A->m * F;
-  EPET_HIP(hipMallocAsync((void**)&x, sizeof(float) * std::max(1ll, nx), st));
-  EPET_HIP(hipMallocAsync((void**)&c, sizeof(float) * std::max(1ll, nx), st));
-  EPET_HIP(hipMallocAsync((void**)&y, sizeof(float) * std::max(1ll, ny), st));
-  EPET_HIP(hipMallocAsync((void**)&e, sizeof(float) * std::max(1ll, ny), st));
-  EPET_HIP(hipMallocAsync((void**)&itp, sizeof(int), st));
+  auto release = [&]() {
+    if (x) (void)hipFreeAsync(x, st);
+    if (y) (void)hipFreeAsync(y, st);
+    if (e) (void)hipFreeAsync(e, st);
+    if (c) (void)hipFreeAsync(c, st);
+    if (itp) (void)hipFreeAsync(itp, st);
+  };
+  if (hipMallocAsync((void**)&x, sizeof(float) * std::max(1ll, nx), st) != hipSuccess ||
+      hipMallocAsync((void**)&c, sizeof(float) * std::max(1ll, nx), st) != hipSuccess ||
+      hipMallocAsync((void**)&y, sizeof(float) * std::max(1ll, ny), st) != hipSuccess ||
+      hipMallocAsync((void**)&e, sizeof(float) * std::max(1ll, ny), st) != hipSuccess ||
+      hipMallocAsync((void**)&itp, sizeof(int), st) != hipSuccess) {
+    (void)hipGetLastError();
+    release();
+    (void)hipStreamSynchronize(st);
+    EPET_FAIL(EPET_ERR_ALLOC, "tuning buffers");
+  }
   k_fill_f32<<<grid_for(nx, 256, 1ll << 31), 256, 0, st>>>(x, nx, 1.0f);
   k_fill_f32<<<grid_for(ny, 256, 1ll << 31), 256, 0, st>>>(y, ny, 1.0f);
-  k_advance<<<1, 64, 0, st>>>(itp);
+  k_fill_i32_one<<<1, 64, 0, st>>>(itp, 0);
   const MfArgs a{F, x, y, e, c, itp, nullptr};
-  A->mf_F = 0;
-  int rc = mf_tune(ctx, A, a, 3);
-  (void)hipFreeAsync(x, st); (void)hipFreeAsync(y, st); (void)hipFreeAsync(e, st);
-  (void)hipFreeAsync(c, st); (void)hipFreeAsync(itp, st);
+  int rc = EPET_OK;
+  bool blocked = false;
+  if (!ctx->opt.batched_gather) {
+    rc = blk_prepare(ctx, A, F);
+    if (rc == EPET_OK && blk_built(A, F)) {
+      rc = blk_tune(ctx, A, a, 3);
+      blocked = true;
+    }
+  }
+  if (rc == EPET_OK && !blocked) {
+    A->mf_F = 0;
+    rc = mf_tune(ctx, A, a, 3);
+  }
+  release();
   EPET_HIP(hipStreamSynchronize(st));
-  if (rc == EPET_OK) {
+  if (rc != EPET_OK) return rc;
+  if (blocked) {
+    int info[9];
+    float t[2];
+    EPET_TRY(blk_info(A, F, info, t));
+    if (cfg) { cfg[0] = info[2]; cfg[1] = info[1]; cfg[2] = info[6]; cfg[3] = info[5]; }
+    if (ms) { ms[0] = t[0]; ms[1] = t[1]; }
+  } else {
     if (cfg) { cfg[0] = A->mf_fwd_R; cfg[1] = A->mf_fwd_P; cfg[2] = A->mf_bwd_R; cfg[3] = A->mf_bwd_P; }
     if (ms) { ms[0] = A->mf_fwd_ms; ms[1] = A->mf_bwd_ms; }
   }
-  return rc;
+  return EPET_OK;
+}
+
+int epet_mlem_kernels(epet_ctx* ctx, const epet_matrix* A, int F, int* info, float* ms) {
+  EPET_CHECK(ctx && A && info, EPET_ERR_ARG, "null argument");
+  for (int i = 0; i < 10; i++) info[i] = 0;
+  float t[2] = {0.0f, 0.0f};
+  if (F >= 2 && A->tr && use_blk(ctx, A, F)) {
+    int bi[9];
+    EPET_TRY(blk_info(A, F, bi, t));
+    info[0] = 2;
+    for (int i = 1; i < 9; i++) info[i] = bi[i];
+  } else if (F >= 2 && A->tr && A->mf_F == F) {
+    info[0] = 1;
+    info[1] = A->mf_fwd_P; info[2] = A->mf_fwd_R; info[5] = A->mf_bwd_P; info[6] = A->mf_bwd_R;
+    t[0] = A->mf_fwd_ms;
+    t[1] = A->mf_bwd_ms;
+  }
+  if (ms) { ms[0] = t[0]; ms[1] = t[1]; }
+  return EPET_OK;
 }
 
 }  // extern "C"

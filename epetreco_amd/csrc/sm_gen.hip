@@ -579,6 +579,15 @@ __global__ __launch_bounds__(256) void k_get_chords(const float* __restrict__ ra
 using namespace epet;
 
 // ====================================================================== C-ABI
+namespace epet {
+// channel id of row i: list[i], or i without a list
+__global__ __launch_bounds__(256) void k_row_ids(const int* __restrict__ list, int m,
+                                                 int* __restrict__ ids) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) ids[i] = list ? list[i] : (int)i;
+}
+}  // namespace epet
+
 extern "C" {
 
 int epet_rays(epet_ctx* ctx, const int* cnl_dev, int n, float* rays_dev) {
@@ -795,6 +804,20 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
     (void)hipFree(rowptr);
     return rc;
   }
+  // channel ids of the rows (the list, or 0..m-1) and the geometry they refer to
+  int* ids = nullptr;
+  if (hipMalloc((void**)&ids, sizeof(int) * (size_t)std::max(1, m)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(col);
+    (void)hipFree(val);
+    (void)hipFree(rowptr);
+    EPET_FAIL(EPET_ERR_ALLOC, "row id allocation failed");
+  }
+  if (m > 0) {
+    k_row_ids<<<grid_for(m, 256, 1ll << 31), 256, 0, ctx->stream>>>(list_dev, m, ids);
+    EPET_HIP(hipGetLastError());
+    EPET_HIP(hipStreamSynchronize(ctx->stream));
+  }
   epet_matrix* A = new epet_matrix();
   A->m = m;
   A->n = g.V;
@@ -803,6 +826,15 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
   A->rowptr = rowptr;
   A->col = col;
   A->val = val;
+  A->row_ids = ids;
+  A->mg.na = g.na;
+  A->mg.n0z = g.n0z;
+  A->mg.n0y = g.n0y;
+  A->mg.n1z = g.n1z;
+  A->mg.n1y = g.n1y;
+  A->mg.nx = g.gn[0];
+  A->mg.ny = g.gn[1];
+  A->mg.nz = g.gn[2];
   A->device = ctx->device;
   *out = A;
   return EPET_OK;

@@ -258,6 +258,13 @@ __global__ void k_set_last_i64(int64_t* p, const int64_t* v) {
   if (threadIdx.x == 0) *p = *v;
 }
 
+__global__ __launch_bounds__(256) void k_gather_i32(const int* __restrict__ src,
+                                                    const int* __restrict__ idx, int n,
+                                                    int* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
 // Structure check of caller arrays: bit 1 = invalid (row pointers not
 // monotone / not spanning [0, nnz], column outside [0, n)); bit 2 = columns not
 // strictly ascending within a row.
@@ -356,10 +363,7 @@ epet_matrix::~epet_matrix() {
   if (lds_val) (void)hipFree(lds_val);
   if (lds_col16) (void)hipFree(lds_col16);
   if (lds_partial) (void)hipFree(lds_partial);
-  if (mfl_off) (void)hipFree(mfl_off);
-  if (mfl_val) (void)hipFree(mfl_val);
-  if (mfl_col16) (void)hipFree(mfl_col16);
-  if (mfl_partial) (void)hipFree(mfl_partial);
+  epet::blk_release(this);
   if (tr) epet_matrix_destroy(tr);
 }
 
@@ -503,6 +507,7 @@ int epet_matrix_destroy(epet_matrix* A) {
     if (A->col) (void)hipFree(A->col);
     if (A->val) (void)hipFree(A->val);
     if (A->rowscale) (void)hipFree(A->rowscale);
+    if (A->row_ids) (void)hipFree(A->row_ids);
   }
   delete A;
   return EPET_OK;
@@ -578,6 +583,17 @@ int epet_matrix_select_rows(epet_ctx* ctx, const epet_matrix* A, const int* rows
                                                   B->col, (uint16_t*)B->val, B->rowscale);
     EPET_HIP(hipGetLastError());
   }
+  if (A->row_ids) {
+    if (hipMalloc((void**)&B->row_ids, sizeof(int) * (size_t)std::max(1, nrows)) != hipSuccess) {
+      (void)hipGetLastError();
+      B->row_ids = nullptr;
+    } else if (nrows) {
+      k_gather_i32<<<grid_for(nrows, 256, 1ll << 31), 256, 0, st>>>(A->row_ids, rows, nrows,
+                                                                    B->row_ids);
+      EPET_HIP(hipGetLastError());
+    }
+    if (B->row_ids) B->mg = A->mg;
+  }
   // last row pointer = nnz: the scan total sits one past the counts
   int64_t* tot = nullptr;
   EPET_HIP(hipMallocAsync((void**)&tot, sizeof(int64_t), st));
@@ -613,6 +629,15 @@ int epet_matrix_convert(epet_ctx* ctx, const epet_matrix* A, int vt, epet_matrix
     k_convert_rows<<<A->m, 256, 0, st>>>(A->rowptr, (const float*)A->val, vt,
                                          (uint16_t*)B->val, B->rowscale);
     EPET_HIP(hipGetLastError());
+  }
+  if (A->row_ids && hipMalloc((void**)&B->row_ids, sizeof(int) * (size_t)std::max(1, A->m)) ==
+                        hipSuccess) {
+    if (A->m)
+      EPET_HIP(hipMemcpyAsync(B->row_ids, A->row_ids, sizeof(int) * A->m, hipMemcpyDeviceToDevice,
+                              st));
+    B->mg = A->mg;
+  } else {
+    (void)hipGetLastError();
   }
   EPET_HIP(hipStreamSynchronize(st));
   *out = B;

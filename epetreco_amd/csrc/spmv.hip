@@ -152,10 +152,7 @@ __global__ __launch_bounds__(256) void k_spmv_n_vec(int m, const int64_t* __rest
     const long long rn = r + nw;
     if (rn < m) pb = row_bounds_load(rowptr, rn, lane);
     float acc = 0.0f;
-    if (b < e)
-      acc = wave_sum(U == 2   ? row_dot_wave<VT>(b, e, lane, col, val, x)
-                     : U == 4 ? row_dot_wave_u<VT, 4>(b, e, lane, col, val, x)
-                              : row_dot_wave_s<VT, U>(b, e, lane, col, val, x));
+    if (b < e) acc = wave_sum(row_dot_wave_s<VT, U>(b, e, lane, col, val, x));
     if (lane == 0) {
       const float v = alpha * (rowscale ? rowscale[r] : 1.0f) * acc;
       y[r] = (beta == 0.0f) ? v : v + beta * y[r];
@@ -307,27 +304,12 @@ template <typename VT, int BWD>
 static int launch_fwd_vec(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha,
                           float beta, float* y) {
   using T = typename VT::T;
-  // measurement knob: EPET_FWD_WG workgroups per CU (default 16)
-  const char* wg = getenv("EPET_FWD_WG");
-  const long long per_cu = wg ? std::max(1, atoi(wg)) : 16;
+  // 16 workgroups per CU, persistent grid-stride over rows; lane-strided
+  // layout with 16 entries per lane per pass
   const long long want = ((long long)A->m + 3) / 4;
-  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, per_cu * ctx->num_cus));
-  // EPET_FWD_U: "2"/"4" quads per lane (vector layout), "8" lane-strided with
-  // 8 entries per lane; default lane-strided with 16 entries per lane
-  const char* u = getenv("EPET_FWD_U");
-  const T* v = (const T*)A->val;
-  if (u && u[0] == '2')
-    k_spmv_n_vec<VT, 2, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
-                                                         x, alpha, beta, y);
-  else if (u && u[0] == '4')
-    k_spmv_n_vec<VT, 4, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
-                                                         x, alpha, beta, y);
-  else if (u && u[0] == '8')
-    k_spmv_n_vec<VT, 8, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
-                                                         x, alpha, beta, y);
-  else
-    k_spmv_n_vec<VT, 16, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, v, A->rowscale,
-                                                          x, alpha, beta, y);
+  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(want, 16ll * ctx->num_cus));
+  k_spmv_n_vec<VT, 16, BWD><<<g, 256, 0, ctx->stream>>>(A->m, A->rowptr, A->col, (const T*)A->val,
+                                                        A->rowscale, x, alpha, beta, y);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
@@ -383,21 +365,11 @@ static int launch_spmv(hipStream_t st, int op, int m, int n, long long nnz, cons
   return EPET_OK;
 }
 
-// Kernel choice: rows of >= 64 entries on average use the vectorised wavefront
-// forward; back projection uses the LDS-tiled kernel once its plan exists
-// (EPET_BP=atomic forces the atomic kernel, for comparison).
-// EPET_BP selects the back projection for measurements: "atomic" (global float
-// atomics), "tiled" (LDS image tiles), "rows" (plain row kernel over the
-// transposed copy); unset: the blocked kernel over the attached transpose when
-// there is one and the values carry no row scale, else the LDS tiles.
-static bool bp_mode(const char* want) {
-  const char* e = getenv("EPET_BP");
-  return e && std::string(e) == want;
-}
-static bool bp_atomic_forced() { return bp_mode("atomic"); }
-
-bool bp_use_transpose(const epet_matrix* A) {
-  return A->tr && !bp_mode("atomic") && !bp_mode("tiled");
+// Back-projection choice: over the attached transpose (LDS-tiled, blocked or
+// row kernel) unless the context asks for the atomic kernel or the LDS image
+// tiles over A itself.
+bool bp_use_transpose(const epet_ctx* ctx, const epet_matrix* A) {
+  return A->tr && ctx->opt.bp != BP_ATOMIC && ctx->opt.bp != BP_TILED;
 }
 
 // Blocked back projection over a transposed copy: the columns of At (channel
@@ -471,7 +443,7 @@ void xb_prepare(epet_ctx* ctx, epet_matrix* At) {
 
 int lds_prepare(epet_ctx* ctx, epet_matrix* A);
 int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
-bool lds_ready(const epet_matrix* A);
+bool lds_ready(const epet_ctx* ctx, const epet_matrix* A);
 int lds_spmv(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha, float beta,
              float* y, bool use_scale, bool bwd);
 
@@ -479,9 +451,9 @@ int lds_spmv(epet_ctx* ctx, const epet_matrix* A, const float* x, float alpha, f
 template <typename VT>
 static int bp_rows(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x, float beta,
                    float* y) {
-  if (lds_ready(At)) return lds_spmv(ctx, At, x, alpha, beta, y, false, true);
+  if (lds_ready(ctx, At)) return lds_spmv(ctx, At, x, alpha, beta, y, false, true);
   const double avg = At->m ? (double)At->nnz / At->m : 0.0;
-  if (At->xb_toff && avg >= 512.0 && !bp_mode("rows")) {
+  if (At->xb_toff && avg >= 512.0 && ctx->opt.bp != BP_ROWS) {
     const unsigned g = (unsigned)(XB_BLOCKS * std::max(1, 2 * ctx->num_cus));
     k_bp_xb<VT, 16><<<g, 256, 0, ctx->stream>>>(At->m, At->rowptr, At->xb_toff, At->col,
                                                  (const typename VT::T*)At->val, x, At->xb_partial);
@@ -501,16 +473,15 @@ static int spmv_dispatch(epet_ctx* ctx, const epet_matrix* A, int op, float alph
   const double avg = A->m ? (double)A->nnz / A->m : 0.0;
   if (op == EPET_OP_N) {
     EPET_TRY(lds_prepare_map(ctx, const_cast<epet_matrix*>(A), true));
-    if (lds_ready(A)) return lds_spmv(ctx, A, x, alpha, beta, y, true, false);
+    if (lds_ready(ctx, A)) return lds_spmv(ctx, A, x, alpha, beta, y, true, false);
   }
-  const char* fu = getenv("EPET_FWD_U");   // "s": the lane-per-entry kernel
-  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0 && !(fu && fu[0] == 's'))
+  if (op == EPET_OP_N && avg >= 64.0 && A->lanes_fwd == 0)
     return launch_fwd_vec<VT, 0>(ctx, A, x, alpha, beta, y);
-  if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(A)) {
+  if (op == EPET_OP_T && !A->rowscale && bp_use_transpose(ctx, A)) {
     EPET_TRY(lds_prepare(ctx, A->tr));
     return bp_rows<VT>(ctx, A->tr, alpha, x, beta, y);
   }
-  if (op == EPET_OP_T && !bp_atomic_forced()) {
+  if (op == EPET_OP_T && ctx->opt.bp != BP_ATOMIC) {
     EPET_TRY(bp_prepare(ctx, const_cast<epet_matrix*>(A)));
     if (A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);
   }

@@ -674,6 +674,128 @@ void eo_mlem_f64_frames(int m, int n, int F, const float* val, const int64_t* ro
   free(tptr); free(trow); free(tval); free(s); free(E); free(C);
 }
 
+/* Static row range [*r0, *r1) of thread t of nt (contiguous, in row order). */
+static void row_range(int m, int t, int nt, int* r0, int* r1) {
+  *r0 = (int)(((int64_t)m * t) / nt);
+  *r1 = (int)(((int64_t)m * (t + 1)) / nt);
+}
+
+/* tests/reco.cu:209-300 loop for F frames (as eo_mlem_f64_frames), with the
+ * back projection c = A^T e scattered into per-thread FP64 images. */
+void eo_mlem_f64_frames_par(int m, int n, int F, const float* val, const int64_t* rowptr,
+                            const int* col, const float* Y, const float* s_in, double* X,
+                            const int* niter_f, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  nthreads = 1;
+#endif
+  const size_t nF = (size_t)(n > 0 ? n : 1) * (size_t)F;
+  double* s = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double ss = 0.0;
+  for (int j = 0; j < n; j++) ss += s_in[j];
+  for (int j = 0; j < n; j++) s[j] = s_in[j] / ss;
+  double* E = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1) * (size_t)F);
+  double* C = (double*)malloc(sizeof(double) * nF);
+  double* priv = (double*)malloc(sizeof(double) * nF * (size_t)nthreads);
+  int maxit = 0;
+  for (int f = 0; f < F; f++) maxit = niter_f[f] > maxit ? niter_f[f] : maxit;
+  for (int it = 0; it < maxit; it++) {
+    /* forward projection and ratio e = y / (A x), 0 where A x = 0 */
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int i = 0; i < m; i++)
+      for (int f0 = 0; f0 < F; f0 += 64) {
+        const int nf = F - f0 < 64 ? F - f0 : 64;
+        double acc[64];
+        for (int f = 0; f < nf; f++) acc[f] = 0.0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const double v = (double)val[k];
+          const double* xr = X + (size_t)col[k] * F + f0;
+          for (int f = 0; f < nf; f++) acc[f] += v * xr[f];
+        }
+        for (int f = 0; f < nf; f++) {
+          const size_t q = (size_t)i * F + f0 + f;
+          E[q] = acc[f] > 0.0 ? (double)Y[q] / acc[f] : 0.0;
+        }
+      }
+    /* back projection: thread t scatters its rows into priv[t], then the
+     * images are summed in thread order */
+#pragma omp parallel num_threads(nthreads)
+    {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      double* cp = priv + nF * (size_t)t;
+      for (size_t q = 0; q < nF; q++) cp[q] = 0.0;
+      int r0, r1;
+      row_range(m, t, nthreads, &r0, &r1);
+      for (int i = r0; i < r1; i++) {
+        const double* er = E + (size_t)i * F;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+          const double v = (double)val[k];
+          double* cr = cp + (size_t)col[k] * F;
+          for (int f = 0; f < F; f++) cr[f] += v * er[f];
+        }
+      }
+#pragma omp barrier
+#pragma omp for schedule(static)
+      for (int j = 0; j < n; j++)
+        for (int f = 0; f < F; f++) {
+          double acc = 0.0;
+          for (int q = 0; q < nthreads; q++) acc += priv[nF * (size_t)q + (size_t)j * F + f];
+          C[(size_t)j * F + f] = acc;
+        }
+    }
+    /* update x <- x c / (s + [s == 0]) and normalisation (reco.cu:273-281) */
+    for (int f = 0; f < F; f++) {
+      if (it >= niter_f[f]) continue;
+      double sum = 0.0;
+      for (int j = 0; j < n; j++) {
+        const double dvs = s[j] + (s[j] == 0.0 ? 1.0 : 0.0);
+        double* xj = &X[(size_t)j * F + f];
+        *xj = *xj * C[(size_t)j * F + f] / dvs;
+        sum += *xj;
+      }
+      for (int j = 0; j < n; j++) X[(size_t)j * F + f] /= sum;
+    }
+  }
+  free(s); free(E); free(C); free(priv);
+}
+
+/* Back projection of an all-ones measurement in FP64 (tests/backprojection.cu:
+ * 115-161 computes it in FP32 through the chunked SpMV^T). */
+void eo_colsum_f64(int m, int n, const float* val, const int64_t* rowptr, const int* col,
+                   double* s, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  nthreads = 1;
+#endif
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  double* priv = (double*)calloc(nn * (size_t)nthreads, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    double* cp = priv + nn * (size_t)t;
+    int r0, r1;
+    row_range(m, t, nthreads, &r0, &r1);
+    for (int i = r0; i < r1; i++)
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) cp[col[k]] += (double)val[k];
+#pragma omp barrier
+#pragma omp for schedule(static)
+    for (int j = 0; j < n; j++) {
+      double acc = 0.0;
+      for (int q = 0; q < nthreads; q++) acc += priv[nn * (size_t)q + j];
+      s[j] = acc;
+    }
+  }
+  free(priv);
+}
+
 void eo_mlem_iter_omp(int m, int n, const float* val, const int64_t* rowptr,
                       const int* col, const float* y, const float* s, float* x,
                       float* yhat, float* c, int nthreads) {

@@ -181,6 +181,20 @@ void eo_mlem_f64(int m, int n, const float* val, const int64_t* rowptr,
 void eo_mlem_f64_frames(int m, int n, int F, const float* val, const int64_t* rowptr,
                         const int* col, const float* Y, const float* s, double* X,
                         const int* niter_f, int nthreads);
+/* eo_mlem_f64_frames for matrices of billions of entries: the back projection
+ * scatters into one private FP64 image per thread (static row ranges) and
+ * sums them in thread order, so no transpose is built.  Deterministic for a
+ * given thread count; equal to eo_mlem_f64_frames up to FP64 summation order
+ * (pinned against it at 1e-12). */
+void eo_mlem_f64_frames_par(int m, int n, int F, const float* val, const int64_t* rowptr,
+                            const int* col, const float* Y, const float* s, double* X,
+                            const int* niter_f, int nthreads);
+/* Sensitivity image s_j = sum_i a_ij accumulated in FP64: the back projection
+ * of an all-ones measurement (tests/backprojection.cu:115-161) that
+ * tests/reco.cu:150-183 reads as its sensitivity.  Per-thread private images
+ * over static row ranges, summed in thread order. */
+void eo_colsum_f64(int m, int n, const float* val, const int64_t* rowptr, const int* col,
+                   double* s, int nthreads);
 /* One FP32 MLEM iteration with OpenMP (the CPU baseline).  Same result as
  * one step of eo_mlem_f32 up to summation order; yhat/e/c scratch provided. */
 void eo_mlem_iter_omp(int m, int n, const float* val, const int64_t* rowptr,

@@ -94,6 +94,11 @@ def lib():
             "eo_mlem_f64_frames": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_float_p,
                                           c_i64_p, c_int_p, c_float_p, c_float_p, c_double_p,
                                           c_int_p, ctypes.c_int]),
+            "eo_mlem_f64_frames_par": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              c_float_p, c_i64_p, c_int_p, c_float_p,
+                                              c_float_p, c_double_p, c_int_p, ctypes.c_int]),
+            "eo_colsum_f64": (None, [ctypes.c_int, ctypes.c_int, c_float_p, c_i64_p, c_int_p,
+                                     c_double_p, ctypes.c_int]),
             "eo_mlem_iter_omp": (None, [ctypes.c_int, ctypes.c_int, c_float_p, c_i64_p,
                                         c_int_p, c_float_p, c_float_p, c_float_p,
                                         c_float_p, c_float_p, ctypes.c_int]),
@@ -307,6 +312,38 @@ def mlem_f64_frames(m, n, val, rowptr, col, Y, s, X0, niter, nthreads=0):
                              ptr(np.ascontiguousarray(s, np.float32), ctypes.c_float),
                              ptr(X, ctypes.c_double), ptr(its, ctypes.c_int), nthreads)
     return X
+
+
+def mlem_f64_frames_par(m, n, val, rowptr, col, Y, s, X0, niter, nthreads=0):
+    """mlem_f64_frames for full-size matrices (per-thread scatter back
+    projection, no transpose); same arguments."""
+    X = np.ascontiguousarray(X0, dtype=np.float64).copy()
+    if X.ndim == 1:
+        X = X[:, None].copy()
+    F = X.shape[1]
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    if Y.ndim == 1:
+        Y = Y[:, None]
+    its = np.broadcast_to(np.asarray(niter, np.int32), (F,)).copy()
+    lib().eo_mlem_f64_frames_par(m, n, F,
+                                 ptr(np.ascontiguousarray(val, np.float32), ctypes.c_float),
+                                 ptr(np.ascontiguousarray(rowptr, np.int64), ctypes.c_int64),
+                                 ptr(np.ascontiguousarray(col, np.int32), ctypes.c_int),
+                                 ptr(np.ascontiguousarray(Y), ctypes.c_float),
+                                 ptr(np.ascontiguousarray(s, np.float32), ctypes.c_float),
+                                 ptr(X, ctypes.c_double), ptr(its, ctypes.c_int), nthreads)
+    return X
+
+
+def colsum_f64(n, val, rowptr, col, nthreads=0):
+    """Sensitivity image sum_i a_ij in FP64 over the rows of a CSR matrix."""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    out = np.zeros(n, dtype=np.float64)
+    lib().eo_colsum_f64(len(rowptr) - 1, n, ptr(np.ascontiguousarray(val, np.float32), ctypes.c_float),
+                        ptr(rowptr, ctypes.c_int64),
+                        ptr(np.ascontiguousarray(col, np.int32), ctypes.c_int),
+                        ptr(out, ctypes.c_double), nthreads)
+    return out
 
 
 def loglik(val, rowptr, col, y, x):

@@ -22,3 +22,15 @@ def decoded(M):
         v = raw.view(np.float16).astype(np.float32)
     rows = np.repeat(np.arange(M.m), np.diff(rp_h))
     return rp_h, col.cpu().numpy(), (v * sc.cpu().numpy()[rows]).astype(np.float32)
+
+
+def env_ctx(monkeypatch, **env):
+    """A fresh Context created with the given EPET_* variables set: a context
+    reads its path choices once, when it is created."""
+    from epetreco_amd.api import Context
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    ctx = Context(0)
+    for k in env:
+        monkeypatch.delenv(k)
+    return ctx

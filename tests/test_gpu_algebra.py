@@ -174,9 +174,9 @@ def test_blocked_transposed_back_projection(gpu_ctx_factory, oracle, monkeypatch
                    beta=beta).cpu().numpy()
         cr = oracle.csrmv(1, m, n, 1.5, val, rp, col, e, beta, y0)
         np.testing.assert_allclose(c, cr, rtol=2e-5, atol=1e-6)
-    monkeypatch.setenv("EPET_BP", "rows")
-    c_rows = A.spmv(OP_T, _t(e, torch.float32)).cpu().numpy()
-    monkeypatch.delenv("EPET_BP")
+    from helpers import env_ctx
+    ctx_rows = env_ctx(monkeypatch, EPET_BP="rows")
+    c_rows = A.spmv(OP_T, _t(e, torch.float32), ctx=ctx_rows).cpu().numpy()
     c_blk = A.spmv(OP_T, _t(e, torch.float32)).cpu().numpy()
     np.testing.assert_allclose(c_blk, c_rows, rtol=2e-6, atol=1e-7)
 
@@ -231,3 +231,19 @@ def test_error_paths_are_loud(gpu_ctx_factory):
         ctx.getChords(torch.zeros(1, 6, device="cuda"), -1)
     with pytest.raises(EpetError, match="transpose"):
         ctx.mlem_tune(A, 8)
+    # host-side argument checks: dtype, frame layout and sizes against the matrix
+    x1 = torch.full((300,), 1.0 / 300, device="cuda")
+    with pytest.raises(EpetError, match="float32"):
+        ctx.mlem(A, y1, s.double(), x1, 1)
+    with pytest.raises(EpetError, match="frames"):
+        ctx.mlem(A, y1, s, torch.full((300, 2), 1.0 / 300, device="cuda"), 1)
+    with pytest.raises(EpetError, match="elements"):
+        ctx.mlem(A, y1[:150].contiguous(), s, x1, 1)
+    with pytest.raises(EpetError, match="elements"):
+        A.spmv(0, torch.ones(200, device="cuda"))
+    with pytest.raises(EpetError, match="elements"):
+        A.spmv(1, torch.ones(300, device="cuda"))
+    with pytest.raises(EpetError, match="elements"):
+        A.spmm(0, torch.ones(200, 4, device="cuda"))
+    with pytest.raises(EpetError, match="frames"):
+        A.spmm(0, torch.ones(300, 4, device="cuda"), torch.zeros(200, 3, device="cuda"))

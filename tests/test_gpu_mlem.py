@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from epetreco_amd import phantom
+from helpers import env_ctx
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +56,21 @@ def test_mlem_small_vs_oracles(small_problem, oracle, use_graph):
     assert (xg >= 0).all()
     # Poisson log-likelihood is nondecreasing under MLEM
     assert np.all(np.diff(ll[:, 0]) >= -1e-9 * np.abs(ll[:-1, 0]))
+
+
+def test_sensitivity_small_vs_oracle(gpu_ctx_factory, oracle):
+    """In-tree sensitivity (the FP64 column sums of the all-channel matrix)
+    against the oracle's FP64 A^T 1 over its own all-channel matrix of the
+    small workload (tests/backprojection.cu:115-161, tests/reco.cu:150-183)."""
+    ctx = gpu_ctx_factory("small", seed=0)
+    s, g = ctx.setup, ctx.grid
+    A = ctx.sm_build(None, m=s.nchannels)
+    s_dev = A.colsum_f64().cpu().numpy()
+    orp, ocol, oval = oracle.sm_rows(s, g, np.arange(s.nchannels, dtype=np.int32),
+                                     ctx.nrays, ctx.seed)
+    s_ref = oracle.colsum_f64(g.nvoxels, oval, orp, ocol)
+    np.testing.assert_allclose(s_dev, s_ref, rtol=1e-12, atol=0)
+    assert np.array_equal(s_dev == 0, s_ref == 0)
 
 
 def test_mlem_identity_one_step(gpu_ctx_factory):
@@ -186,11 +202,10 @@ def test_single_frame_transpose_path(small_problem, oracle, storage, monkeypatch
     V = A.n
     e = torch.rand(A.m, device="cuda")
     c_tr = A.spmv(OP_T, e)
-    monkeypatch.setenv("EPET_BP", "tiled")
-    c_tiled = A.spmv(OP_T, e)
+    ctx_t = env_ctx(monkeypatch, EPET_BP="tiled")
+    c_tiled = A.spmv(OP_T, e, ctx=ctx_t)
     x_t = torch.full((V,), 1.0 / V, device="cuda")
-    ctx.mlem(A, prob.y, prob.s, x_t, 10, use_graph=True)
-    monkeypatch.delenv("EPET_BP")
+    ctx_t.mlem(A, prob.y, prob.s, x_t, 10, use_graph=True)
     assert rel_l2(c_tr.cpu().numpy(), c_tiled.cpu().numpy().astype(np.float64)) < 1e-6
     x = torch.full((V,), 1.0 / V, device="cuda")
     ctx.mlem(A, prob.y, prob.s, x, 10, use_graph=True)
@@ -233,31 +248,29 @@ def test_lds_tiled_products_and_mlem(gpu_ctx_factory, oracle, storage, monkeypat
         np.testing.assert_allclose(c, oracle.csrmv(1, m, n, 1.0, vq, rq, cq, e, 0.0,
                                                    np.zeros(n, np.float32)), rtol=2e-5, atol=1e-6)
     # the tile-major LDS product against the row kernel over the original arrays
-    monkeypatch.setenv("EPET_LDS", "0")
-    y_rows = A.spmv(OP_N, torch.as_tensor(x, device="cuda")).cpu().numpy()
-    monkeypatch.delenv("EPET_LDS")
+    ctx_rows = env_ctx(monkeypatch, EPET_LDS="0")
+    y_rows = A.spmv(OP_N, torch.as_tensor(x, device="cuda"), ctx=ctx_rows).cpu().numpy()
     np.testing.assert_allclose(y, y_rows, rtol=1e-6, atol=1e-7)
     counts = rng.poisson(30.0, m).astype(np.float32)
     s = torch.as_tensor(np.bincount(cq, weights=vq, minlength=n).astype(np.float32), device="cuda")
     yy = torch.as_tensor(counts, device="cuda")
     x1 = torch.full((n,), 1.0 / n, device="cuda")
     ctx.mlem(A, yy, s, x1, 8, use_graph=True)
-    monkeypatch.setenv("EPET_LDS", "0")
     x2 = torch.full((n,), 1.0 / n, device="cuda")
-    ctx.mlem(A, yy, s, x2, 8, use_graph=True)
-    monkeypatch.delenv("EPET_LDS")
+    ctx_rows.mlem(A, yy, s, x2, 8, use_graph=True)
     assert rel_l2(x1.cpu().numpy(), x2.cpu().numpy().astype(np.float64)) < 1e-5
     x64 = oracle.mlem_f64(m, n, vq, rq, cq, counts, s.cpu().numpy(), np.full(n, 1.0 / n), 8)
     assert rel_l2(x1.cpu().numpy(), x64) < REL_L2
 
 
-@pytest.mark.parametrize("F,P", [(5, "4"), (16, "8"), (13, "8")])
-def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
-    """Multi-frame LDS-tiled MLEM (P frames per pass, ragged frame blocks when P
-    does not divide F) on long rows over 70,001 columns (several column tiles
-    each way), FP32 values: images after 6 iterations vs the gather kernels
-    (rel-L2 <= 1e-5) and vs the multi-frame FP64 oracle (<= 1e-4), and the
-    per-frame log-likelihood of the last iteration vs the gather kernels."""
+@pytest.mark.parametrize("F", [5, 13, 16, 32])
+def test_multiframe_blocked_path(gpu_ctx_factory, oracle, monkeypatch, F):
+    """Multi-frame MLEM through the blocked register-accumulated products
+    (padded frame counts 8, 16 and 32; several column tiles each way; the back
+    projection split over workgroups) on long rows over 70,001 columns, FP32
+    values: images after 6 iterations vs the gather kernels (rel-L2 <= 1e-5)
+    and vs the multi-frame FP64 oracle (<= 1e-4), and the per-frame
+    log-likelihood of the last iteration vs the gather kernels."""
     from epetreco_amd.api import Matrix
     ctx = gpu_ctx_factory()
     rng = np.random.default_rng(F)
@@ -271,14 +284,16 @@ def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
     Y = rng.poisson(20.0, (m, F)).astype(np.float32)
     s = torch.as_tensor(np.bincount(col, weights=val, minlength=n).astype(np.float32), device="cuda")
     Yd = torch.as_tensor(Y, device="cuda")
-    monkeypatch.setenv("EPET_MFL_P", P)
     X1 = torch.full((n, F), 1.0 / n, device="cuda")
+    _say("blocked multi-frame MLEM, F =", F)
     ll1, _ = ctx.mlem(A, Yd, s, X1, 6, use_graph=True)
-    monkeypatch.setenv("EPET_MFL", "0")
+    cfg = ctx.batched_config(A, F)
+    _say("  ", cfg)
+    assert cfg["path"].startswith("blocked")
+    ctx_g = env_ctx(monkeypatch, EPET_BATCHED="gather")
     X2 = torch.full((n, F), 1.0 / n, device="cuda")
-    ll2, _ = ctx.mlem(A, Yd, s, X2, 6, use_graph=True)
-    monkeypatch.delenv("EPET_MFL")
-    monkeypatch.delenv("EPET_MFL_P")
+    ll2, _ = ctx_g.mlem(A, Yd, s, X2, 6, use_graph=True)
+    assert ctx_g.batched_config(A, F)["path"].startswith("gather")
     X64 = oracle.mlem_f64_frames(m, n, val, rp, col, Y, s.cpu().numpy(), np.full((n, F), 1.0 / n), 6)
     for f in range(F):
         assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
@@ -287,33 +302,28 @@ def test_multiframe_lds_path(gpu_ctx_factory, oracle, monkeypatch, F, P):
 
 
 def test_voxel_renumbered_tiles(gpu_ctx_factory, monkeypatch):
-    """Tile-major copies in y-major voxel order (EPET_LDS_MAP=1, for matrices
-    over the context's grid): the projector workload's forward product (single
-    frame) and a 16-frame MLEM match the copies in the matrix's own column
+    """Single-frame tile-major copies in y-major voxel order (EPET_LDS_MAP=1,
+    for matrices over the context's grid): the projector workload's forward
+    product and 4 MLEM iterations match the copies in the matrix's own column
     order and the kernels without LDS copies."""
-    from epetreco_amd.api import OP_N
+    from epetreco_amd.api import Matrix, OP_N
     monkeypatch.setenv("EPET_LDS_MAP", "1")
     ctx = gpu_ctx_factory("projector", seed=0)
+    monkeypatch.delenv("EPET_LDS_MAP")
     A = ctx.sm_build(None, m=ctx.setup.nchannels)
     rng = np.random.default_rng(3)
     x = torch.as_tensor(rng.random(A.n).astype(np.float32), device="cuda")
     y_map = A.spmv(OP_N, x).cpu().numpy()
-    monkeypatch.setenv("EPET_LDS", "0")
-    y_rows = A.spmv(OP_N, x).cpu().numpy()
-    monkeypatch.delenv("EPET_LDS")
+    ctx_rows = env_ctx(monkeypatch, EPET_LDS="0")
+    y_rows = A.spmv(OP_N, x, ctx=ctx_rows).cpu().numpy()
     np.testing.assert_allclose(y_map, y_rows, rtol=1e-6, atol=1e-7)
-    A.attach_transpose()
-    F = 16
-    Y = torch.as_tensor(rng.poisson(5.0, (A.m, F)).astype(np.float32), device="cuda")
+    Y = torch.as_tensor(rng.poisson(5.0, A.m).astype(np.float32), device="cuda")
     s = A.colsum_f64().float()
-    X1 = torch.full((A.n, F), 1.0 / A.n, device="cuda")
-    ctx.mlem(A, Y, s, X1, 4, use_graph=True)
-    from epetreco_amd.api import Matrix
+    x1 = torch.full((A.n,), 1.0 / A.n, device="cuda")
+    ctx.mlem(A, Y, s, x1, 4, use_graph=True)
     rp, col, val, _ = A.to_torch()
-    B = Matrix.from_csr(ctx, rp, col, val, A.n).attach_transpose()
-    monkeypatch.setenv("EPET_LDS_MAP", "0")
-    X2 = torch.full((A.n, F), 1.0 / A.n, device="cuda")
-    ctx.mlem(B, Y, s, X2, 4, use_graph=True)
-    monkeypatch.delenv("EPET_LDS_MAP")
-    for f in range(F):
-        assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
+    B = Matrix.from_csr(ctx, rp, col, val, A.n)
+    ctx_plain = env_ctx(monkeypatch, EPET_LDS_MAP="0")
+    x2 = torch.full((A.n,), 1.0 / A.n, device="cuda")
+    ctx_plain.mlem(B, Y, s, x2, 4, use_graph=True)
+    assert rel_l2(x1.cpu().numpy(), x2.cpu().numpy().astype(np.float64)) < 1e-5

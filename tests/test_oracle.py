@@ -515,6 +515,50 @@ def test_mlem_frames_matches_single_frame_bitwise(oracle):
         assert np.array_equal(X[:, f], xf), f
 
 
+def test_mlem_frames_par_matches_transpose_path(oracle):
+    """The full-size FP64 oracle (per-thread scatter back projection) equals
+    the bitwise-pinned transpose form up to FP64 summation order, for several
+    thread counts, frames frozen at their own counts and an empty column."""
+    rng = np.random.default_rng(8)
+    m, n, F = 1500, 120, 4
+    M, rp, col, val = _rand_csr(rng, m, n, 0.1)
+    keep = col != 33
+    M[:, 33] = 0
+    rp = np.concatenate([[0], np.cumsum((M != 0).sum(1))]).astype(np.int64)
+    col, val = col[keep], val[keep]
+    Y = rng.poisson(20, (m, F)).astype(np.float32)
+    s = M.sum(0).astype(np.float32)
+    X0 = rng.random((n, F)) + 0.1
+    its = np.array([0, 3, 7, 5], np.int32)
+    Xa = oracle.mlem_f64_frames(m, n, val, rp, col, Y, s, X0, its, nthreads=2)
+    for nt in (1, 3, 5):
+        Xb = oracle.mlem_f64_frames_par(m, n, val, rp, col, Y, s, X0, its, nthreads=nt)
+        np.testing.assert_allclose(Xb, Xa, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(Xa[:, 0], X0[:, 0])   # zero iterations: unchanged
+
+
+def test_colsum_f64_closed_form(oracle):
+    """FP64 sensitivity image = dense column sums (all-ones back projection);
+    an empty column gives exactly 0; the thread count changes nothing beyond
+    FP64 summation order."""
+    rng = np.random.default_rng(9)
+    m, n = 900, 75
+    M, rp, col, val = _rand_csr(rng, m, n, 0.2)
+    keep = col != 5
+    M[:, 5] = 0
+    rp = np.concatenate([[0], np.cumsum((M != 0).sum(1))]).astype(np.int64)
+    col, val = col[keep], val[keep]
+    ref = M.astype(np.float64).sum(0)
+    for nt in (1, 4):
+        cs = oracle.colsum_f64(n, val, rp, col, nthreads=nt)
+        np.testing.assert_allclose(cs, ref, rtol=1e-13)
+        assert cs[5] == 0.0
+    ones = np.ones(m, np.float32)
+    # the same image as the oracle's transposed product with a ones vector
+    np.testing.assert_allclose(cs, oracle.csrmv(1, m, n, 1.0, val, rp, col, ones, 0.0,
+                                                np.zeros(n, np.float32)), rtol=2e-6)
+
+
 # ------------------------------------------------------------------ chunk helpers
 def test_chunk_helpers(oracle):
     """wrappers.hpp:405-445 semantics: full chunks, remainder, zero past the end."""
