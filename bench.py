@@ -158,6 +158,10 @@ def main():
                          "(0: skip)")
     ap.add_argument("--cpu-generation", type=int, default=1,
                     help="time the oracle's generation on a channel sample (0: skip)")
+    ap.add_argument("--profile-batched", default="",
+                    help="after building the problem, convert to this storage (bf16, f16, f32), "
+                         "tune the batched products for --frames frames and run 3 batched MLEM "
+                         "iterations (for kernel traces and counter runs)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="after building the problem, launch only the forward SpMV, the back "
                          "projection and a 1 GiB device copy (3 times each) for counter runs")
@@ -203,6 +207,23 @@ def main():
     torch.cuda.synchronize()
     tr_s = time.time() - t0
     say(f"transposed copy built in {tr_s:.2f} s")
+
+    if args.profile_batched:
+        F = args.frames
+        vt = {"bf16": VAL_BF16, "f16": VAL_F16, "f32": None}[args.profile_batched]
+        M = A if vt is None else A.convert(vt)
+        M.attach_transpose()
+        Yp = torch.poisson(torch.full((m, F), 3.0, device="cuda")).contiguous()
+        Xp = torch.full((V, F), 1.0 / V, device="cuda")
+        t0 = time.perf_counter()
+        cfg, tms = ctx.mlem_tune(M, F)
+        torch.cuda.synchronize()
+        say(f"tuned in {time.perf_counter() - t0:.2f} s (incl. blocked copies): {cfg} {tms}")
+        ctx.mlem(M, Yp, prob.s, Xp, 3, use_graph=False)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_batched": args.profile_batched, "frames": F,
+                          "config": ctx.batched_config(M, F)}))
+        return
 
     if args.profile_kernels:
         xv = torch.rand(V, device="cuda")

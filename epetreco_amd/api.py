@@ -250,8 +250,9 @@ class Context:
         return ch, vid, cnt
 
     def mlem_tune(self, A: "Matrix", frames: int):
-        """Autotune the multi-frame kernels for `frames`; returns
-        ((R_fwd, P_fwd, R_bwd, P_bwd), (ms_fwd, ms_bwd))."""
+        """Autotune the multi-frame products for `frames`; returns
+        ((rb_fwd, P_fwd, rb_bwd, P_bwd), (ms_fwd, ms_bwd)) for the blocked
+        products, ((R_fwd, P_fwd, R_bwd, P_bwd), ...) for the gather kernels."""
         cfg = np.zeros(4, dtype=np.int32)
         ms = np.zeros(2, dtype=np.float32)
         check(lib().epet_mlem_tune(self._h, A.handle, int(frames),
@@ -269,14 +270,12 @@ class Context:
                                       ms.ctypes.data_as(_lib.c_float_p)))
         path = {2: "blocked", 1: "gather", 0: "untuned"}[int(info[0])]
         if path == "blocked":
-            return {"path": "blocked (register-accumulated, LDS-staged tiles, k_blk)",
-                    "forward": {"frames_per_lane": int(info[1]), "row_groups_per_lane": int(info[2]),
-                                "rows_per_workgroup": int(info[3]), "splits": int(info[4]),
-                                "tuned_ms": float(ms[0])},
-                    "back_projection": {"frames_per_lane": int(info[5]),
-                                        "row_groups_per_lane": int(info[6]),
-                                        "rows_per_workgroup": int(info[7]), "splits": int(info[8]),
-                                        "tuned_ms": float(ms[1])}}
+            def shape(o, t):
+                return {"frames_per_lane": int(info[o]), "lanes_per_row_slot": int(info[o + 1]),
+                        "rows_per_block": int(info[o + 2]), "splits": int(info[o + 3]),
+                        "tuned_ms": float(ms[t])}
+            return {"path": "blocked (LDS-staged column tiles, entry-balanced slots, k_blk)",
+                    "forward": shape(1, 0), "back_projection": shape(5, 1)}
         if path == "gather":
             return {"path": "gather kernels (k_mf_fwd / k_mf_bwd)",
                     "forward": {"frame_tile": int(info[1]), "rows_per_wavefront": int(info[2]),

@@ -16,17 +16,22 @@
 //     seg_base[h (nG+1) + j]: first entry of granule j (BLK_G row positions) in
 //     tile h, so that a row block's entries in a tile are one contiguous range.
 //
-// A workgroup owns a row block and keeps its rows' F-frame sums in registers
-// while it walks the column tiles: for every tile the block touches it stages
-// the tile's C columns x Fp frames of G in LDS, and each lane accumulates P
-// frames of K rows (L = Fp / P lanes per row).  The matrix streams once, no
-// partial sums per (row, tile) reach memory, and the row's complete sums feed
+// A workgroup owns a row block and keeps its rows' F-frame sums in LDS while
+// it walks the column tiles: for every tile the block touches it stages the
+// tile's C columns x Fp frames of G in LDS, then splits the block's entries in
+// that tile evenly over its slots of L lanes (L = Fp / P, P frames per lane):
+// each slot walks a contiguous run of entry quads, accumulates in registers
+// and adds a row's partial sum into the LDS row sums (LDS float atomics) when
+// the row or the run ends.  Only a quarter of a block's rows meet a given tile,
+// so entries, not rows, are what is balanced.  The matrix streams once, no
+// partial sums per (row, tile) reach memory, and the complete row sums feed
 // the epilogue: the MLEM ratio and the FP64 log-likelihood for the forward
 // projection, the image for the back projection.  When a direction has few
 // row blocks (the back projection's voxel rows), each block's tiles are split
 // by entry count over `splits` workgroups that write partial sums, which a
 // reduction adds in split order.  The grid is persistent (one workgroup per CU
-// looping over the work items), and the (K, P) shape is autotuned per matrix.
+// looping over the work items), and the (rows per block, frames per lane)
+// shape is autotuned per matrix and frame count.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -44,9 +49,13 @@ namespace epet {
 
 constexpr int BLK_THREADS = 1024;
 constexpr int BLK_WAVES = BLK_THREADS / WAVE;
-constexpr int BLK_TILE_FLOATS = 32768;      // 128 KB of the gathered vector per tile
+constexpr int BLK_TILE_FLOATS = 16384;      // 64 KB of the gathered vector per tile
 constexpr int BLK_G = 64;                   // row granule of the segment bases
-constexpr int BLK_MAX_RB = 4 * BLK_THREADS; // rows per block (one scan pass)
+constexpr int BLK_MAX_RB = BLK_THREADS;     // rows per block (one row per thread in the scan)
+constexpr int BLK_LDS_LIMIT = 160 * 1024 - 2048;   // dynamic LDS per workgroup (gfx950)
+// entry quads in flight per slot (fewer with 8 frames per lane, whose
+// gathered rows take twice the registers)
+template <int P> constexpr int blk_u() { return P == 4 ? 4 : 2; }
 
 // Blocked copy of one matrix for one padded frame count.
 struct BlkCopy {
@@ -63,8 +72,9 @@ struct BlkCopy {
   long long nent = 0;
   float* partial = nullptr;    // splits x m_pad x Fp
   long long partial_len = 0;
-  // launch shape (autotuned): frames per lane, row groups per lane, splits
-  int P = 4, K = 4, splits = 1;
+  float* gp = nullptr;         // the gathered vector in column-position order, T C x Fp
+  // launch shape (autotuned): frames per lane, rows per block, splits
+  int P = 4, rb = 256, splits = 1;
   int tuned_F = 0;
   float ms = 0.0f;
   BlkCopy* next = nullptr;     // copies for other padded frame counts
@@ -76,6 +86,7 @@ struct BlkCopy {
     if (ent) (void)hipFree(ent);
     if (eval) (void)hipFree(eval);
     if (partial) (void)hipFree(partial);
+    if (gp) (void)hipFree(gp);
     delete next;
   }
 };
@@ -88,28 +99,30 @@ void blk_release(epet_matrix* A) {
 struct BlkArgs {
   int F, Fp, lgFp, C, T, m_pad, nG;
   int rb, nblocks, splits;
-  int L, S;                    // lanes per row, row slots per wavefront
+  int L, S;                    // lanes per row slot, row slots per wavefront
+  int lgNS;                    // log2 of the slots per workgroup
   int mode;                    // 0: forward MLEM ratio, 1: image, 2: partial sums
-  int F4;                      // G rows are float4-aligned (F % 4 == 0)
   const int* rowperm;
-  const int* colinv;
   const int64_t* seg_base;
   const uint16_t* cnt;
   const void* ent;
   const float* eval;
-  const float* G;              // gathered vector, n x F (frame-minor)
-  float* out;                  // forward: E (m x F); back projection: C (n x F)
+  long long zero_quad;         // index of four zero entries after the stream
+  const float* Gp;             // gathered vector in column-position order, T C x Fp
+  float* out;                  // forward: ratio in position order (m_pad x Fp);
+                               // back projection: image (n x F, frame-minor)
   float* partial;
   const float* rowscale;       // forward: per-row scale of the stored values
-  int fold;                    // forward: E carries the row scale
+  int fold;                    // forward: the ratio carries the row scale
   const float* Y;              // forward: data (m x F)
   const int* itp;
   double* loglik;
 };
 
 // ------------------------------------------------------------------ entries
-// Four consecutive entries (16-byte aligned): tile-local columns (masked to the
-// tile, which also bounds every LDS index) and values.
+// Four consecutive entries (16-byte aligned): the columns as byte offsets of
+// their frames in the 64 KB LDS tile (the 16-bit mask bounds every LDS address)
+// and the values.
 template <typename VT> struct BlkEnt;
 template <> struct BlkEnt<ValBF16> {
   static __device__ __forceinline__ void load(const BlkArgs& a, long long e, unsigned cm,
@@ -144,253 +157,264 @@ template <> struct BlkEnt<ValF32> {
   }
 };
 
-// ------------------------------------------------------------------ pieces
-// Tile h of G into LDS: xs[v Fp + f] = G[colinv[h C + v] F + f] (0 for padding
-// positions and frames past F).
-__device__ __forceinline__ void blk_stage(const BlkArgs& a, int h, float* xs) {
-  const int fq = a.Fp >> 2;
-  const int lgfq = a.lgFp - 2;
-  const int n4 = a.C << lgfq;
-  const int* ci = a.colinv + (long long)h * a.C;
-  for (int i = threadIdx.x; i < n4; i += BLK_THREADS) {
-    const int v = i >> lgfq, f = (i & (fq - 1)) << 2;
-    const int c = ci[v];
-    float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (c >= 0 && f < a.F) {
-      const float* src = a.G + (long long)c * a.F + f;
-      if (a.F4) {
-        val = *reinterpret_cast<const float4*>(src);
-      } else {
-        val.x = src[0];
-        if (f + 1 < a.F) val.y = src[1];
-        if (f + 2 < a.F) val.z = src[2];
-        if (f + 3 < a.F) val.w = src[3];
-      }
-    }
-    reinterpret_cast<float4*>(xs)[i] = val;
-  }
-}
-
-// Segment starts of the block's rbe row positions in tile h, relative to the
-// block's first entry in the tile: s_start[r] (exclusive), s_start[rbe] = total.
-// Part 1 (before a barrier) scans within wavefronts; part 2 adds the wavefront
-// bases.
-__device__ __forceinline__ void blk_starts1(const BlkArgs& a, int h, int p0, int rbe,
-                                            int* s_start, int* s_wsum, int loc[4], int* tot) {
-  const int t = threadIdx.x;
-  const int r = 4 * t;
-  const uint16_t* cn = a.cnt + (long long)h * a.m_pad + p0;
-  int v[4] = {0, 0, 0, 0};
-  if (r < rbe) {
-    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
-    const us4 c4 = *reinterpret_cast<const us4*>(cn + r);   // rbe is a multiple of 64
-    v[0] = c4.x; v[1] = c4.y; v[2] = c4.z; v[3] = c4.w;
-  }
-  loc[0] = 0;
-  loc[1] = v[0];
-  loc[2] = v[0] + v[1];
-  loc[3] = v[0] + v[1] + v[2];
-  const int mine = loc[3] + v[3];
-  int x = mine;
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    const int y = __shfl_up(x, off, WAVE);
-    if (lane >= off) x += y;
-  }
-  if (lane == WAVE - 1) s_wsum[wave_id()] = x;
-  *tot = x - mine;   // exclusive within the wavefront
-}
-
-__device__ __forceinline__ void blk_starts2(int rbe, int* s_start, const int* s_wsum,
-                                            const int loc[4], int excl) {
-  const int t = threadIdx.x;
-  int base = excl;
-  const int w = wave_id();
-  int all = 0;
-#pragma unroll
-  for (int k = 0; k < BLK_WAVES; k++) {
-    const int c = s_wsum[k];
-    base += k < w ? c : 0;
-    all += c;
-  }
-  const int r = 4 * t;
-  if (r < rbe) {
-    s_start[r] = base + loc[0];
-    s_start[r + 1] = base + loc[1];
-    s_start[r + 2] = base + loc[2];
-    s_start[r + 3] = base + loc[3];
-  }
-  if (t == 0) s_start[rbe] = all;
-}
-
-// Tile range [h0, h1) of split sp of a block: tiles whose entry prefix (over
-// the block's chunks, in tile order) falls in [tot sp / splits, tot (sp+1) /
-// splits).  Wavefront 0 computes it; the caller holds barriers around it.
-__device__ __forceinline__ void blk_split_range(const BlkArgs& a, int j0, int j1, int sp,
-                                                int* s_range) {
-  if (wave_id() != 0) return;
-  const int lane = lane_id();
-  const int per = (a.T + WAVE - 1) / WAVE;
-  const int hb = min(a.T, lane * per), he = min(a.T, hb + per);
-  long long mine = 0;
-  for (int h = hb; h < he; h++) {
-    const int64_t* sb = a.seg_base + (long long)h * (a.nG + 1);
-    mine += sb[j1] - sb[j0];
-  }
-  long long x = mine;
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    const long long y = __shfl_up(x, off, WAVE);
-    if (lane >= off) x += y;
-  }
-  const long long tot = __shfl(x, WAVE - 1, WAVE);
-  const long long lo = tot * sp / a.splits, hi = tot * (sp + 1) / a.splits;
-  if (lane == 0) { s_range[0] = a.T; s_range[1] = a.T; }
-  __builtin_amdgcn_wave_barrier();
-  long long pre = x - mine;   // entries before tile hb
-  for (int h = hb; h < he; h++) {
-    const int64_t* sb = a.seg_base + (long long)h * (a.nG + 1);
-    // the prefix is nondecreasing: the smallest tile whose prefix reaches lo
-    // (resp. hi) starts the split (resp. the next one)
-    if (pre >= lo) atomicMin(&s_range[0], h);
-    if (pre >= hi) atomicMin(&s_range[1], h);
-    pre += sb[j1] - sb[j0];
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0 && sp == a.splits - 1) s_range[1] = a.T;
-  if (lane == 0 && sp == 0) s_range[0] = 0;
-}
-
 // ------------------------------------------------------------------ main kernel
-template <typename VT, int P, int K>
+// Dynamic LDS of a workgroup (all of it dynamic, so the tile starts at LDS
+// address 0 and an entry's byte offset is its LDS address):
+//   tile      C Fp floats (64 KB) of Gp
+//   acc       rb Fp floats: the block's row sums
+//   nz_row    rb ints, nz_start rb + 1 ints: rows with entries in the tile
+//   tl_c0     BLK_THREADS int64, tl_h BLK_THREADS ints: non-empty tiles of a
+//             chunk of BLK_THREADS tiles and the block's first entry in each
+//   wsum      2 x 16 ints, cnt 1 int; ll 64 doubles
+constexpr int BLK_LDS_TAIL = sizeof(int64_t) * BLK_THREADS + sizeof(int) * BLK_THREADS +
+                             sizeof(int) * (2 * BLK_WAVES + 2) + sizeof(double) * 64;
+__host__ __device__ inline size_t blk_lds_bytes(int C, int Fp, int rb) {
+  return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) + sizeof(int) * (2 * (size_t)rb + 2) +
+         BLK_LDS_TAIL;
+}
+
+// Workgroup-wide exclusive scan of (a, b) pairs in thread order: returns this
+// thread's prefixes and the totals.  Two barriers.
+__device__ __forceinline__ void blk_scan2(int xa, int xb, int* wsum, int* pa, int* pb, int* ta,
+                                          int* tb) {
+  const int lane = lane_id(), w = wave_id();
+  int ya = xa, yb = xb;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const int za = __shfl_up(ya, off, WAVE), zb = __shfl_up(yb, off, WAVE);
+    if (lane >= off) { ya += za; yb += zb; }
+  }
+  if (lane == WAVE - 1) { wsum[w] = ya; wsum[BLK_WAVES + w] = yb; }
+  __syncthreads();
+  int ba = 0, bb = 0, sa = 0, sb = 0;
+#pragma unroll
+  for (int v = 0; v < BLK_WAVES; v++) {
+    const int ca = wsum[v], cb = wsum[BLK_WAVES + v];
+    ba += v < w ? ca : 0;
+    bb += v < w ? cb : 0;
+    sa += ca;
+    sb += cb;
+  }
+  __syncthreads();
+  *pa = ba + ya - xa;
+  *pb = bb + yb - xb;
+  *ta = sa;
+  *tb = sb;
+}
+
+template <typename VT, int P>
 __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   extern __shared__ float4 blk_lds[];
   float* xs = reinterpret_cast<float*>(blk_lds);
-  int* s_start = reinterpret_cast<int*>(xs + ((size_t)a.C << a.lgFp));
-  __shared__ int s_wsum[BLK_WAVES];
-  __shared__ int s_range[2];
-  __shared__ double s_ll[64];
+  float* acc = xs + ((size_t)a.C << a.lgFp);
+  int* nz_row = reinterpret_cast<int*>(acc + ((size_t)a.rb << a.lgFp));
+  int* nz_start = nz_row + a.rb;
+  int64_t* tl_c0 = reinterpret_cast<int64_t*>(nz_start + a.rb + 2);
+  int* tl_h = reinterpret_cast<int*>(tl_c0 + BLK_THREADS);
+  int* wsum = tl_h + BLK_THREADS;
+  double* s_ll = reinterpret_cast<double*>(wsum + 2 * BLK_WAVES + 2);
+  const int t = threadIdx.x;
   const int lane = lane_id();
   const int w = wave_id();
   const int slot = lane / a.L, q = lane % a.L;
-  const unsigned cm = (unsigned)a.C - 1u;
-  if (threadIdx.x < 64) s_ll[threadIdx.x] = 0.0;
+  const int ks = w * a.S + slot;           // this lane's slot
+  const unsigned cm = 0xFFFFu;
+  const int rbfp = a.rb << a.lgFp;
+  const int tile4 = (a.C << a.lgFp) >> 2;  // float4 per tile (4096)
+  const unsigned lane_off = (unsigned)(q * P * 4);
+  const char* xb = reinterpret_cast<const char*>(xs);
+  if (t < 64) s_ll[t] = 0.0;
+  double ll = 0.0;                         // log-likelihood of frame t mod Fp
   const long long nwork = (long long)a.nblocks * a.splits;
   for (long long work = blockIdx.x; work < nwork; work += gridDim.x) {
     const int blk = (int)(work / a.splits), sp = (int)(work % a.splits);
     const int p0 = blk * a.rb;
     const int rbe = min(a.rb, a.m_pad - p0);
     const int j0 = p0 / BLK_G, j1 = (p0 + rbe) / BLK_G;
-    int h0 = 0, h1 = a.T;
-    if (a.splits > 1) {
+    // this work item's tiles: an equal share of the tile range
+    const int hlo = (int)((long long)a.T * sp / a.splits);
+    const int hhi = (int)((long long)a.T * (sp + 1) / a.splits);
+    __syncthreads();                       // the previous block's epilogue is done
+    for (int i = t; i < rbfp; i += BLK_THREADS) acc[i] = 0.0f;
+    for (int hb = hlo; hb < hhi; hb += BLK_THREADS) {
+      // the block's non-empty tiles among the next BLK_THREADS, in order, with
+      // its first entry in each
+      const int hh = hb + t;
+      long long c0h = 0;
+      int fl = 0;
+      if (hh < hhi) {
+        const int64_t* sb = a.seg_base + (long long)hh * (a.nG + 1);
+        c0h = sb[j0];
+        fl = sb[j1] > c0h ? 1 : 0;
+      }
+      int pk, pz, ntl, dz;
+      __syncthreads();                     // the previous chunk's list is no longer read
+      blk_scan2(fl, 0, wsum, &pk, &pz, &ntl, &dz);
+      if (fl) {
+        tl_h[pk] = hh;
+        tl_c0[pk] = c0h;
+      }
       __syncthreads();
-      blk_split_range(a, j0, j1, sp, s_range);
-      __syncthreads();
-      h0 = s_range[0];
-      h1 = s_range[1];
-    }
-    float acc[K][P];
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-      for (int j = 0; j < P; j++) acc[k][j] = 0.0f;
-    for (int h = h0; h < h1; h++) {
-      const int64_t* sb = a.seg_base + (long long)h * (a.nG + 1);
-      const long long c0 = sb[j0], c1 = sb[j1];
-      if (c0 == c1) continue;   // the block has no entry in this tile
-      __syncthreads();          // the previous tile's readers are done
-      blk_stage(a, h, xs);
-      int loc[4], excl;
-      blk_starts1(a, h, p0, rbe, s_start, s_wsum, loc, &excl);
-      __syncthreads();
-      blk_starts2(rbe, s_start, s_wsum, loc, excl);
-      __syncthreads();
-      const float* xq = xs + q * P;
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int r = (w * K + k) * a.S + slot;
-        long long e = 0, en = 0;
-        if (r < rbe) {
-          e = c0 + s_start[r];
-          en = c0 + s_start[r + 1];
+      if (ntl == 0) continue;
+      // the first tile's slice of Gp and row counts, requested ahead
+      const float4* g4 = reinterpret_cast<const float4*>(a.Gp);
+      long long gb = (long long)tl_h[0] * tile4 + t;
+      float4 g0 = g4[gb], g1 = g4[gb + BLK_THREADS], g2 = g4[gb + 2 * BLK_THREADS],
+             g3 = g4[gb + 3 * BLK_THREADS];
+      int cn = t < rbe ? (int)a.cnt[(long long)tl_h[0] * a.m_pad + p0 + t] : 0;
+      for (int it = 0; it < ntl; it++) {
+        const long long c0 = tl_c0[it];
+        __syncthreads();                   // the previous tile's readers are done
+        float4* x4 = reinterpret_cast<float4*>(xs);
+        x4[t] = g0;
+        x4[t + BLK_THREADS] = g1;
+        x4[t + 2 * BLK_THREADS] = g2;
+        x4[t + 3 * BLK_THREADS] = g3;
+        // the compacted list of the block's rows with entries in this tile and
+        // their segment starts
+        int pc, pn, tc, tn;
+        blk_scan2(cn, cn > 0 ? 1 : 0, wsum, &pc, &pn, &tc, &tn);
+        if (cn > 0) {
+          nz_row[pn] = t;
+          nz_start[pn] = pc;
         }
+        if (t == 0) nz_start[tn] = tc;
+        // the next tile's slice and counts are requested before this tile's
+        // work (the last tile requests its own again)
+        {
+          const int hn = tl_h[min(it + 1, ntl - 1)];
+          gb = (long long)hn * tile4 + t;
+          g0 = g4[gb];
+          g1 = g4[gb + BLK_THREADS];
+          g2 = g4[gb + 2 * BLK_THREADS];
+          g3 = g4[gb + 3 * BLK_THREADS];
+          cn = t < rbe ? (int)a.cnt[(long long)hn * a.m_pad + p0 + t] : 0;
+        }
+        __syncthreads();
+        // this slot's run of entry quads: an equal share of the tile's entries
+        // (NS slots, a power of two; nq NS < 2^31 for tiles of 64 KB)
+        const int nq = tc >> 2;
+        int e = ((nq * ks) >> a.lgNS) << 2;
+        const int e_end = ((nq * (ks + 1)) >> a.lgNS) << 2;
+        int kk = 0;
+        if (e < e_end) {
+          // the compacted row holding entry e: the last one starting at or before e
+          int lo = 0, hi = tn - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (nz_start[mid] <= e) lo = mid; else hi = mid - 1;
+          }
+          kk = lo;
+        }
+        int row = nz_row[min(kk, max(tn - 1, 0))];
+        int rend = nz_start[min(kk + 1, tn)];
+        float a4[P];
+#pragma unroll
+        for (int j = 0; j < P; j++) a4[j] = 0.0f;
         for (;;) {
-          const bool act = e < en;
-          if (!__any(act)) break;
-          if (act) {
-            // two 4-entry groups in flight; the second is zeroed past the segment
-            const bool two = e + 4 < en;
-            unsigned c[8];
-            float v[8];
-            BlkEnt<VT>::load(a, e, cm, c, v);
-            BlkEnt<VT>::load(a, two ? e + 4 : e, cm, c + 4, v + 4);
+          if (!__any(e < e_end)) break;
+          // BLK_U entry quads of the run requested at once; quads past the run
+          // read the zero quad stored after the entries
+          constexpr int BLK_U = blk_u<P>();
+          unsigned c[BLK_U][4];
+          float v[BLK_U][4];
 #pragma unroll
-            for (int t = 4; t < 8; t++) v[t] = two ? v[t] : 0.0f;
+          for (int u = 0; u < BLK_U; u++) {
+            const long long eu = e + 4 * u < e_end ? c0 + e + 4 * u : a.zero_quad;
+            BlkEnt<VT>::load(a, eu, cm, c[u], v[u]);
+          }
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-              const float4* xp = reinterpret_cast<const float4*>(xq + (c[t] << a.lgFp));
+          for (int u = 0; u < BLK_U; u++) {
+            // the quad's four gathered rows requested together, then accumulated
+            float4 xv[4][P / 4];
+#pragma unroll
+            for (int t4 = 0; t4 < 4; t4++) {
+              const float4* xp = reinterpret_cast<const float4*>(xb + (c[u][t4] | lane_off));
+#pragma unroll
+              for (int i = 0; i < P / 4; i++) xv[t4][i] = xp[i];
+            }
+#pragma unroll
+            for (int t4 = 0; t4 < 4; t4++)
 #pragma unroll
               for (int i = 0; i < P / 4; i++) {
-                const float4 xv = xp[i];
-                acc[k][4 * i + 0] += v[t] * xv.x;
-                acc[k][4 * i + 1] += v[t] * xv.y;
-                acc[k][4 * i + 2] += v[t] * xv.z;
-                acc[k][4 * i + 3] += v[t] * xv.w;
+                a4[4 * i + 0] += v[u][t4] * xv[t4][i].x;
+                a4[4 * i + 1] += v[u][t4] * xv[t4][i].y;
+                a4[4 * i + 2] += v[u][t4] * xv[t4][i].z;
+                a4[4 * i + 3] += v[u][t4] * xv[t4][i].w;
+              }
+            if (e < e_end) {
+              e += 4;
+              if (e == rend || e == e_end) {
+                // the row (or this slot's run) ends: add its partial sums
+                float* dst = acc + ((size_t)row << a.lgFp) + q * P;
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                  atomicAdd(dst + j, a4[j]);
+                  a4[j] = 0.0f;
+                }
+                if (e == rend && e < e_end) {
+                  kk++;
+                  row = nz_row[kk];
+                  rend = nz_start[kk + 1];
+                }
               }
             }
           }
-          e += 8;
         }
       }
     }
+    __syncthreads();
     // ---- epilogue: the block's rows are complete (or this split's share is)
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int r = (w * K + k) * a.S + slot;
-      if (r >= rbe) continue;
-      const int p = p0 + r;
-      if (a.mode == 2) {
-        float* dst = a.partial + (((long long)sp * a.m_pad + p) << a.lgFp) + q * P;
-#pragma unroll
-        for (int i = 0; i < P / 4; i++)
-          reinterpret_cast<float4*>(dst)[i] =
-              make_float4(acc[k][4 * i], acc[k][4 * i + 1], acc[k][4 * i + 2], acc[k][4 * i + 3]);
-        continue;
-      }
-      const int orig = a.rowperm[p];
+    if (a.mode == 2) {
+      float* dst = a.partial + (((long long)sp * a.m_pad + p0) << a.lgFp);
+      for (int i = t; i < (rbe << a.lgFp) / 4; i += BLK_THREADS)
+        reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(acc)[i];
+      continue;
+    }
+    for (int i = t; i < (rbe << a.lgFp); i += BLK_THREADS) {
+      const int r = i >> a.lgFp, f = i & (a.Fp - 1);
+      if (f >= a.F) continue;
+      const int orig = a.rowperm[p0 + r];
       if (orig < 0) continue;
       if (a.mode == 1) {
-#pragma unroll
-        for (int j = 0; j < P; j++) {
-          const int f = q * P + j;
-          if (f < a.F) a.out[(long long)orig * a.F + f] = acc[k][j];
-        }
+        a.out[(long long)orig * a.F + f] = acc[i];
       } else {
         const float sc = a.rowscale ? a.rowscale[orig] : 1.0f;
-#pragma unroll
-        for (int j = 0; j < P; j++) {
-          const int f = q * P + j;
-          if (f >= a.F) continue;
-          const long long o = (long long)orig * a.F + f;
-          const float yh = acc[k][j] * sc;
-          const float yi = a.Y[o];
-          const float ev = yh > 0.0f ? yi / yh : 0.0f;
-          a.out[o] = a.fold ? ev * sc : ev;
-          if (yh > 0.0f && a.loglik) atomicAdd(&s_ll[f], (double)yi * log((double)yh) - (double)yh);
-        }
+        const float yh = acc[i] * sc;
+        const float yi = a.Y[(long long)orig * a.F + f];
+        const float ev = yh > 0.0f ? yi / yh : 0.0f;
+        a.out[((long long)(p0 + r) << a.lgFp) + f] = a.fold ? ev * sc : ev;
+        if (yh > 0.0f) ll += (double)yi * log((double)yh) - (double)yh;
       }
     }
   }
   if (a.mode == 0 && a.loglik) {
+    // thread t only ever saw frame t mod Fp
     __syncthreads();
-    if (threadIdx.x < a.F) unsafeAtomicAdd(&a.loglik[(long long)(*a.itp) * a.F + threadIdx.x],
-                                           s_ll[threadIdx.x]);
+    atomicAdd(&s_ll[t & (a.Fp - 1)], ll);
+    __syncthreads();
+    if (t < a.F) unsafeAtomicAdd(&a.loglik[(long long)(*a.itp) * a.F + t], s_ll[t]);
   }
 }
 
-// Split reductions, in split order.  Forward: the MLEM ratio and the FP64
-// log-likelihood (one atomic per workgroup and frame); back projection: the image.
+// Gp[pos Fp + f] = G[colinv[pos] F + f] (0 for padding positions and frames
+// past F): the gathered vector in the copy's column order, so that a tile is
+// one contiguous 64 KB read.
+__global__ __launch_bounds__(256) void k_blk_gather(float* __restrict__ Gp,
+                                                    const float* __restrict__ G,
+                                                    const int* __restrict__ colinv, long long npos,
+                                                    int F, int lgFp) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (npos << lgFp)) return;
+  const long long pos = i >> lgFp;
+  const int f = (int)(i & ((1 << lgFp) - 1));
+  const int c = colinv[pos];
+  Gp[i] = (c >= 0 && f < F) ? G[(long long)c * F + f] : 0.0f;
+}
+
+// Split reductions, in split order.  Forward: the MLEM ratio (in row-position
+// order) and the FP64 log-likelihood (one atomic per workgroup and frame); back
+// projection: the image.
 __global__ __launch_bounds__(256) void k_blk_reduce(BlkArgs a) {
   __shared__ double s_ll[64];
   if (threadIdx.x < 64) s_ll[threadIdx.x] = 0.0;
@@ -413,7 +437,7 @@ __global__ __launch_bounds__(256) void k_blk_reduce(BlkArgs a) {
       const float yh = s * sc;
       const float yi = a.Y[o];
       const float ev = yh > 0.0f ? yi / yh : 0.0f;
-      a.out[o] = a.fold ? ev * sc : ev;
+      a.out[(p << a.lgFp) + f] = a.fold ? ev * sc : ev;
       if (yh > 0.0f && a.loglik) atomicAdd(&s_ll[f], (double)yi * log((double)yh) - (double)yh);
     }
   }
@@ -457,7 +481,7 @@ __global__ __launch_bounds__(256) void k_blk_count(int m_pad, int T, int lgC,
 // column order (stable: ranks by ballot within 64-entry chunks), then zero
 // entries up to the padded length.
 template <typename V>
-__global__ __launch_bounds__(256) void k_blk_fill(int m_pad, int T, int lgC,
+__global__ __launch_bounds__(256) void k_blk_fill(int m_pad, int T, int lgC, int lgFp,
                                                   const int* __restrict__ rowperm,
                                                   const int64_t* __restrict__ rowptr,
                                                   const int* __restrict__ col,
@@ -491,7 +515,9 @@ __global__ __launch_bounds__(256) void k_blk_fill(int m_pad, int T, int lgC,
         if (act && h == key) {
           const int rank = __popcll(mask & ((1ull << lane) - 1ull));
           const long long dst = off[(long long)key * m_pad + p] + start + rank;
-          const unsigned local = (unsigned)pos & cm;
+          // the entry's column as the byte offset of its frames in the LDS tile
+          // (C Fp floats = 64 KB, so it fits 16 bits)
+          const unsigned local = ((unsigned)pos & cm) << (lgFp + 2);
           if (sizeof(V) == 2) {
             static_cast<uint32_t*>(ent)[dst] = local | ((uint32_t)(uint16_t)val[k] << 16);
           } else {
@@ -522,6 +548,15 @@ __global__ __launch_bounds__(256) void k_blk_fill(int m_pad, int T, int lgC,
       }
     }
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__global__ void k_blk_zero_quad(void* ent, float* eval, long long at, int eb) {
+  const int i = threadIdx.x;
+  if (i < 4) {
+    if (eb == 4) static_cast<uint32_t*>(ent)[at + i] = 0u;
+    else static_cast<uint16_t*>(ent)[at + i] = 0;
+    if (eval) eval[at + i] = 0.0f;
   }
 }
 
@@ -643,7 +678,9 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
   int* cpos_d = nullptr;
   int* cnt32 = nullptr;
   int64_t* off = nullptr;
-  bool ok = hipMalloc((void**)&B->rowperm, sizeof(int) * (size_t)B->m_pad) == hipSuccess &&
+  const long long ngp = ((long long)B->T * B->C) << B->lgFp;
+  bool ok = hipMalloc((void**)&B->gp, sizeof(float) * (size_t)ngp) == hipSuccess &&
+            hipMalloc((void**)&B->rowperm, sizeof(int) * (size_t)B->m_pad) == hipSuccess &&
             hipMalloc((void**)&B->colinv, sizeof(int) * ci.size()) == hipSuccess &&
             hipMalloc((void**)&cpos_d, sizeof(int) * cpos.size()) == hipSuccess &&
             hipMalloc((void**)&B->seg_base, sizeof(int64_t) * (size_t)B->T * (B->nG + 1)) ==
@@ -666,6 +703,7 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
   EPET_HIP(hipMemcpyAsync(B->colinv, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, st));
   EPET_HIP(hipMemcpyAsync(cpos_d, cpos.data(), sizeof(int) * cpos.size(), hipMemcpyHostToDevice, st));
   k_fill_i32_v<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(cnt32, nseg, 0);
+  k_fill_i32_v<<<grid_for(ngp, 256, 1ll << 31), 256, 0, st>>>(reinterpret_cast<int*>(B->gp), ngp, 0);
   const size_t lds = sizeof(int) * (256 / WAVE) * (size_t)B->T;
   int rc = EPET_OK;
   if (lds > 160 * 1024) {
@@ -696,20 +734,22 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
         B->T, B->nG, B->m_pad, off, B->seg_base);
     (void)hipFree(cnt32);
     cnt32 = nullptr;
+    // entries, then one zero quad that reads past a run are pointed at
     const size_t eb = M->value_type == EPET_VAL_F32 ? 2 : 4;
-    if (hipMalloc(&B->ent, eb * (size_t)std::max(4ll, tot)) != hipSuccess ||
+    if (hipMalloc(&B->ent, eb * (size_t)(tot + 4)) != hipSuccess ||
         (M->value_type == EPET_VAL_F32 &&
-         hipMalloc((void**)&B->eval, sizeof(float) * (size_t)std::max(4ll, tot)) != hipSuccess)) {
+         hipMalloc((void**)&B->eval, sizeof(float) * (size_t)(tot + 4)) != hipSuccess)) {
       (void)hipGetLastError();
       cleanup();
       delete B;
       return EPET_OK;
     }
+    k_blk_zero_quad<<<1, 64, 0, st>>>(B->ent, B->eval, tot, (int)eb);
     if (M->value_type == EPET_VAL_F32)
-      k_blk_fill<float><<<g, 256, lds, st>>>(B->m_pad, B->T, B->lgC, B->rowperm, M->rowptr, M->col,
+      k_blk_fill<float><<<g, 256, lds, st>>>(B->m_pad, B->T, B->lgC, B->lgFp, B->rowperm, M->rowptr, M->col,
                                              (const float*)M->val, cpos_d, off, B->ent, B->eval);
     else
-      k_blk_fill<uint16_t><<<g, 256, lds, st>>>(B->m_pad, B->T, B->lgC, B->rowperm, M->rowptr,
+      k_blk_fill<uint16_t><<<g, 256, lds, st>>>(B->m_pad, B->T, B->lgC, B->lgFp, B->rowperm, M->rowptr,
                                                 M->col, (const uint16_t*)M->val, cpos_d, off,
                                                 B->ent, nullptr);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
@@ -727,40 +767,26 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
 // ------------------------------------------------------------------ launch
 typedef void (*BlkKern)(BlkArgs);
 
-template <typename VT>
-static BlkKern blk_kernel(int P, int K) {
-#define EPET_BK(PP, KK) \
-  if (P == PP && K == KK) return k_blk<VT, PP, KK>;
-  EPET_BK(4, 1) EPET_BK(4, 2) EPET_BK(4, 4) EPET_BK(4, 8)
-  EPET_BK(8, 1) EPET_BK(8, 2) EPET_BK(8, 4)
-  EPET_BK(16, 1) EPET_BK(16, 2)
-  EPET_BK(32, 1)
-#undef EPET_BK
-  return nullptr;
-}
-
-static BlkKern blk_kernel_vt(int vt, int P, int K) {
-  switch (vt) {
-    case EPET_VAL_F32: return blk_kernel<ValF32>(P, K);
-    case EPET_VAL_BF16: return blk_kernel<ValBF16>(P, K);
-    default: return blk_kernel<ValF16>(P, K);
+static BlkKern blk_kernel_vt(int vt, int P) {
+#define EPET_BK(VTT)                          \
+  switch (P) {                                \
+    case 4: return k_blk<VTT, 4>;             \
+    case 8: return k_blk<VTT, 8>;             \
+    default: return nullptr;                  \
   }
+  switch (vt) {
+    case EPET_VAL_F32: EPET_BK(ValF32)
+    case EPET_VAL_BF16: EPET_BK(ValBF16)
+    default: EPET_BK(ValF16)
+  }
+#undef EPET_BK
 }
 
-// rows per block of a (P, K) shape for Fp padded frames
-static int blk_rb(int Fp, int P, int K) {
-  const int L = Fp / P;
-  return BLK_WAVES * (WAVE / L) * K;
-}
-
-// K P accumulators per lane: at most 32, so that 1024-thread workgroups keep
-// within 128 VGPRs without spilling
-static bool blk_shape_valid(int Fp, int P, int K) {
-  if (P > Fp || K * P > 32) return false;
-  const int L = Fp / P;
-  if (L > WAVE) return false;
-  const int rb = blk_rb(Fp, P, K);
-  return rb % BLK_G == 0 && rb <= BLK_MAX_RB;
+// A (frames per lane, rows per block) shape fits when the lanes per slot
+// divide a wavefront and the workgroup's LDS holds the tile and the row sums.
+static bool blk_shape_valid(const BlkCopy* B, int P, int rb) {
+  if (P > B->Fp || B->Fp / P > WAVE || rb % BLK_G != 0 || rb > BLK_MAX_RB) return false;
+  return blk_lds_bytes(B->C, B->Fp, rb) <= (size_t)BLK_LDS_LIMIT;
 }
 
 // work-item splits of a direction: enough work items for every CU twice over
@@ -786,12 +812,17 @@ static int blk_partial_ensure(BlkCopy* B, int splits) {
   return EPET_OK;
 }
 
-// One product with an explicit shape (mode 0 forward MLEM step, 1 image).
-static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int K, int splits, int F,
-                      const float* G, float* out, int mode, const float* rowscale, int fold,
-                      const float* Y, const int* itp, double* loglik) {
-  BlkKern kern = blk_kernel_vt(B->vt, P, K);
-  EPET_CHECK(kern, EPET_ERR_STATE, "blocked products: no kernel for this shape");
+// One product with an explicit shape over B's position-ordered gathered
+// vector B->gp (mode 0: forward MLEM step writing the ratio in row-position
+// order to `out`; mode 1: image).
+static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits, int F,
+                      float* out, int mode, const float* rowscale, int fold, const float* Y,
+                      const int* itp, double* loglik) {
+  BlkKern kern = blk_kernel_vt(B->vt, P);
+  EPET_CHECK(kern && blk_shape_valid(B, P, rb), EPET_ERR_STATE,
+             "blocked products: no kernel for this shape");
+  EPET_CHECK(splits <= 1 || (B->partial && B->partial_len >= (long long)splits * B->m_pad * B->Fp),
+             EPET_ERR_STATE, "blocked products: partial buffer missing");
   BlkArgs a{};
   a.F = F;
   a.Fp = B->Fp;
@@ -800,20 +831,20 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int K, int splits,
   a.T = B->T;
   a.m_pad = B->m_pad;
   a.nG = B->nG;
-  a.rb = blk_rb(B->Fp, P, K);
-  a.nblocks = (B->m_pad + a.rb - 1) / a.rb;
+  a.rb = rb;
+  a.nblocks = (B->m_pad + rb - 1) / rb;
   a.splits = splits;
   a.L = B->Fp / P;
   a.S = WAVE / a.L;
+  a.lgNS = ilog2(BLK_WAVES * a.S);
   a.mode = splits > 1 ? 2 : mode;
-  a.F4 = (F % 4 == 0) ? 1 : 0;
   a.rowperm = B->rowperm;
-  a.colinv = B->colinv;
   a.seg_base = B->seg_base;
   a.cnt = B->cnt;
   a.ent = B->ent;
   a.eval = B->eval;
-  a.G = G;
+  a.zero_quad = B->nent;
+  a.Gp = B->gp;
   a.out = out;
   a.partial = B->partial;
   a.rowscale = rowscale;
@@ -821,25 +852,34 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int K, int splits,
   a.Y = Y;
   a.itp = itp;
   a.loglik = loglik;
-  EPET_CHECK(splits <= 1 || (B->partial && B->partial_len >= (long long)splits * B->m_pad * B->Fp),
-             EPET_ERR_STATE, "blocked products: partial buffer missing");
-  const size_t lds = sizeof(float) * ((size_t)B->C << B->lgFp) + sizeof(int) * (BLK_MAX_RB + 4);
-  static bool attr_set[2][3][64] = {};
-  const int pi = ilog2(P), ki = ilog2(K);
-  bool& done = attr_set[0][B->vt][pi * 8 + ki];
+  // every kernel may take the largest dynamic LDS of a workgroup (set once)
+  static bool attr_set[3][4] = {};
+  bool& done = attr_set[B->vt][ilog2(P)];
   if (!done) {
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+                              BLK_LDS_LIMIT);
     done = true;
   }
   const long long nwork = (long long)a.nblocks * a.splits;
   const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(nwork, ctx->num_cus));
-  kern<<<grid, BLK_THREADS, lds, ctx->stream>>>(a);
+  kern<<<grid, BLK_THREADS, blk_lds_bytes(B->C, B->Fp, rb), ctx->stream>>>(a);
   if (splits > 1) {
     a.mode = mode;
     k_blk_reduce<<<grid_for((long long)B->m_pad * F, 256, 16ll * ctx->num_cus), 256, 0,
                    ctx->stream>>>(a);
   }
+  EPET_HIP(hipGetLastError());
+  return EPET_OK;
+}
+
+// The forward copy's gathered vector (the image) in its column order.  The
+// back projection's gathered vector (the ratio) is written in that order by the
+// forward product itself: the forward copy's row order is the transposed
+// copy's column order.
+static int blk_gather_x(epet_ctx* ctx, const BlkCopy* f, int F, const float* x) {
+  const long long npos = (long long)f->T * f->C;
+  k_blk_gather<<<grid_for(npos << f->lgFp, 256, 1ll << 31), 256, 0, ctx->stream>>>(
+      f->gp, x, f->colinv, npos, F, f->lgFp);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
 }
@@ -913,16 +953,17 @@ int blk_iteration(epet_ctx* ctx, const epet_matrix* A, const MfArgs& a) {
   const BlkCopy* f = blk_find(A, Fp);
   const BlkCopy* b = blk_find(A->tr, Fp);
   EPET_CHECK(f && b, EPET_ERR_STATE, "blocked copies missing");
-  EPET_TRY(blk_launch(ctx, f, f->P, f->K, f->splits, a.F, a.x, a.e, 0, A->rowscale,
+  EPET_TRY(blk_gather_x(ctx, f, a.F, a.x));
+  EPET_TRY(blk_launch(ctx, f, f->P, f->rb, f->splits, a.F, b->gp, 0, A->rowscale,
                       A->rowscale ? 1 : 0, a.y, a.itp, a.loglik));
-  EPET_TRY(blk_launch(ctx, b, b->P, b->K, b->splits, a.F, a.e, a.c, 1, nullptr, 0, nullptr,
+  EPET_TRY(blk_launch(ctx, b, b->P, b->rb, b->splits, a.F, a.c, 1, nullptr, 0, nullptr,
                       nullptr, nullptr));
   return EPET_OK;
 }
 
-// Autotuner: every valid (frames per lane P, row groups per lane K) shape of
-// each direction is timed with HIP events on the context's stream (reps
-// launches after one warm-up) and the fastest kept.
+// Autotuner: every valid (frames per lane P, rows per block rb) shape of each
+// direction is timed with HIP events on the context's stream (reps launches
+// after one warm-up) and the fastest kept.
 int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
   const int Fp = padded_frames(a.F);
   BlkCopy* dirs[2] = {blk_find(A, Fp), blk_find(A->tr, Fp)};
@@ -930,28 +971,35 @@ int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
   hipEvent_t e0, e1;
   EPET_HIP(hipEventCreate(&e0));
   EPET_HIP(hipEventCreate(&e1));
-  static const int Ps[4] = {4, 8, 16, 32};
-  static const int Ks[4] = {1, 2, 4, 8};
+  // frames per lane: 4 (a slot of Fp/4 lanes reads whole voxel rows, no bank
+  // conflicts) or 8; 16 and 32 lost to bank conflicts at every block size
+  // (53 and 86 ms against 35 ms per 32-frame forward product at 52^3)
+  EPET_TRY(blk_gather_x(ctx, dirs[0], a.F, a.x));
+  static const int Ps[2] = {4, 8};
+  static const int RBs[3] = {128, 256, 512};
   int rc = EPET_OK;
   for (int d = 0; d < 2 && rc == EPET_OK; d++) {
     BlkCopy* B = dirs[d];
     float best = 1e30f;
-    int bP = 0, bK = 0, bS = 1;
-    for (int pi = 0; pi < 4 && rc == EPET_OK; pi++)
-      for (int ki = 0; ki < 4 && rc == EPET_OK; ki++) {
-        const int P = Ps[pi], K = Ks[ki];
-        if (!blk_shape_valid(Fp, P, K)) continue;
-        const int splits = blk_splits(B, blk_rb(Fp, P, K), ctx->num_cus);
+    int bP = 0, bR = 0, bS = 1;
+    for (int pi = 0; pi < 2 && rc == EPET_OK; pi++)
+      for (int ri = 0; ri < 3 && rc == EPET_OK; ri++) {
+        const int P = Ps[pi], rb = RBs[ri];
+        if (!blk_shape_valid(B, P, rb)) continue;
+        const int splits = blk_splits(B, rb, ctx->num_cus);
         if (blk_partial_ensure(B, splits) != EPET_OK) { (void)hipGetLastError(); continue; }
         auto run = [&]() {
-          return d == 0 ? blk_launch(ctx, B, P, K, splits, a.F, a.x, a.e, 0, A->rowscale,
+          return d == 0 ? blk_launch(ctx, B, P, rb, splits, a.F, dirs[1]->gp, 0, A->rowscale,
                                      A->rowscale ? 1 : 0, a.y, a.itp, nullptr)
-                        : blk_launch(ctx, B, P, K, splits, a.F, a.e, a.c, 1, nullptr, 0, nullptr,
+                        : blk_launch(ctx, B, P, rb, splits, a.F, a.c, 1, nullptr, 0, nullptr,
                                      nullptr, nullptr);
         };
         rc = run();
         if (rc != EPET_OK) break;
-        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "event"); break; }
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
+          rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "event");
+          break;
+        }
         for (int r = 0; r < reps && rc == EPET_OK; r++) rc = run();
         if (rc != EPET_OK) break;
         float ms = 0.0f;
@@ -961,16 +1009,18 @@ int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
           break;
         }
         ms /= reps;
-        if (ms < best) { best = ms; bP = P; bK = K; bS = splits; }
+        if (ms < best) { best = ms; bP = P; bR = rb; bS = splits; }
       }
     if (rc == EPET_OK) {
-      EPET_CHECK(bP > 0, EPET_ERR_STATE, "blocked products: no valid shape");
-      B->P = bP;
-      B->K = bK;
-      B->splits = bS;
-      B->ms = best;
-      B->tuned_F = a.F;
-      rc = blk_partial_ensure(B, bS);
+      if (bP == 0) rc = fail(EPET_ERR_STATE, __FILE__, __LINE__, "blocked products: no valid shape");
+      else {
+        B->P = bP;
+        B->rb = bR;
+        B->splits = bS;
+        B->ms = best;
+        B->tuned_F = a.F;
+        rc = blk_partial_ensure(B, bS);
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -978,9 +1028,10 @@ int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
   return rc;
 }
 
-// Launch shape of the blocked products for F frames: {1, P_fwd, K_fwd,
-// rows per block fwd, splits fwd, P_bwd, K_bwd, rows per block bwd, splits bwd}
-// and the tuned times, or {0} when the blocked path is not in use.
+// Launch shape of the blocked products for F frames: {1, P_fwd, lanes per
+// slot fwd, rows per block fwd, splits fwd, P_bwd, lanes per slot bwd, rows
+// per block bwd, splits bwd} and the tuned times, or {0} when the blocked path
+// is not in use.
 int blk_info(const epet_matrix* A, int F, int info[9], float ms[2]) {
   for (int i = 0; i < 9; i++) info[i] = 0;
   ms[0] = ms[1] = 0.0f;
@@ -989,8 +1040,8 @@ int blk_info(const epet_matrix* A, int F, int info[9], float ms[2]) {
   const BlkCopy* f = blk_find(A, Fp);
   const BlkCopy* b = blk_find(A->tr, Fp);
   info[0] = 1;
-  info[1] = f->P; info[2] = f->K; info[3] = blk_rb(Fp, f->P, f->K); info[4] = f->splits;
-  info[5] = b->P; info[6] = b->K; info[7] = blk_rb(Fp, b->P, b->K); info[8] = b->splits;
+  info[1] = f->P; info[2] = Fp / f->P; info[3] = f->rb; info[4] = f->splits;
+  info[5] = b->P; info[6] = Fp / b->P; info[7] = b->rb; info[8] = b->splits;
   ms[0] = f->ms;
   ms[1] = b->ms;
   return EPET_OK;

@@ -613,7 +613,7 @@ int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms) {
     int info[9];
     float t[2];
     EPET_TRY(blk_info(A, F, info, t));
-    if (cfg) { cfg[0] = info[2]; cfg[1] = info[1]; cfg[2] = info[6]; cfg[3] = info[5]; }
+    if (cfg) { cfg[0] = info[3]; cfg[1] = info[1]; cfg[2] = info[7]; cfg[3] = info[5]; }
     if (ms) { ms[0] = t[0]; ms[1] = t[1]; }
   } else {
     if (cfg) { cfg[0] = A->mf_fwd_R; cfg[1] = A->mf_fwd_P; cfg[2] = A->mf_bwd_R; cfg[3] = A->mf_bwd_P; }

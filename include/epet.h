@@ -246,21 +246,22 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int nframes, const float* y_d
 
 /* Autotuner of the multi-frame products epet_mlem runs for F frames on A and
  * its attached transpose.  Blocked products (the default): every shape of
- * (row groups per lane K in {1,2,4,8}) x (frames per lane P in {4,8,16,32}),
- * K P <= 32 (accumulator registers), is timed per direction and the fastest kept; cfg = {K_fwd,
- * P_fwd, K_bwd, P_bwd}.  Gather kernels (EPET_BATCHED=gather, or blocked
- * copies that cannot be allocated): (rows per wavefront R in {1,2,4,8}) x
- * (frames per lane tile P in {4,8,16,32}); cfg = {R_fwd, P_fwd, R_bwd, P_bwd}.
- * cfg (4 ints) and ms (2 floats: forward, back projection) may be NULL.
- * epet_mlem runs it on first use for a frame count.  No reference counterpart:
- * the reference's loop is single-frame (tests/reco.cu:232-250). */
+ * (rows per block rb in {128,256,512}) x (frames per lane P in {4,8}) that
+ * fits the workgroup's LDS is timed per direction and the fastest kept;
+ * cfg = {rb_fwd, P_fwd, rb_bwd, P_bwd}.  Gather kernels
+ * (EPET_BATCHED=gather, or blocked copies that cannot be allocated): (rows per
+ * wavefront R in {1,2,4,8}) x (frames per lane tile P in {4,8,16,32});
+ * cfg = {R_fwd, P_fwd, R_bwd, P_bwd}.  cfg (4 ints) and ms (2 floats: forward,
+ * back projection) may be NULL.  epet_mlem runs it on first use for a frame
+ * count.  No reference counterpart: the reference's loop is single-frame
+ * (tests/reco.cu:232-250). */
 int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms);
 
 /* The multi-frame path epet_mlem runs for F frames on A, as tuned: info (10
- * ints) = {path, P_fwd, K_or_R_fwd, rows per workgroup fwd, splits fwd, P_bwd,
- * K_or_R_bwd, rows per workgroup bwd, splits bwd, 0}; path 2 = blocked
- * products, 1 = gather kernels, 0 = not tuned yet.  ms (2 floats, may be NULL)
- * = the tuned times per launch. */
+ * ints).  Blocked products: {2, P_fwd, lanes per row slot fwd, rows per block
+ * fwd, splits fwd, P_bwd, lanes bwd, rows per block bwd, splits bwd, 0}.
+ * Gather kernels: {1, P_fwd, R_fwd, 0, 0, P_bwd, R_bwd, 0, 0, 0}.  {0, ...}:
+ * not tuned yet.  ms (2 floats, may be NULL) = the tuned times per launch. */
 int epet_mlem_kernels(epet_ctx* ctx, const epet_matrix* A, int F, int* info, float* ms);
 
 /* ---- chunk helpers (wrappers.hpp:405-445) ---------------------------------------- */

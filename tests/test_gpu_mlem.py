@@ -167,17 +167,26 @@ def test_graph_replays_match_eager(small_problem):
     assert rel_l2(xg.cpu().numpy(), xe.cpu().numpy().astype(np.float64)) < 1e-5
 
 
-def test_multiframe_autotuner(small_problem):
-    """The tuner times every (rows per wavefront, frame tile) pair and keeps
-    valid choices; results with the tuned kernels match the default ones."""
+def test_multiframe_autotuner(small_problem, monkeypatch):
+    """The tuner times every valid shape of the path epet_mlem runs and keeps a
+    valid one: (rows per block, frames per lane) for the blocked products,
+    (rows per wavefront, frame tile) for the gather kernels; results with the
+    tuned kernels match the default ones."""
     ctx, prob = small_problem
     from epetreco_amd.api import Matrix
     rp, col, val, _ = prob.A.to_torch()
     A = Matrix.from_csr(ctx, rp, col, val, prob.A.n).attach_transpose()
-    (Rf, Pf, Rb, Pb), (mf, mb) = ctx.mlem_tune(A, 16)
-    _say("tuned fwd R,P =", Rf, Pf, f"{mf:.3f} ms; bwd R,P =", Rb, Pb, f"{mb:.3f} ms")
-    assert Rf in (1, 2, 4, 8) and Pf in (4, 8, 16, 32) and Pf <= 64 // Rf
-    assert Rb in (1, 2, 4, 8) and Pb in (4, 8, 16, 32) and Pb <= 64 // Rb
+    (rf, Pf, rbk, Pb), (mf, mb) = ctx.mlem_tune(A, 16)
+    _say("blocked: fwd rb,P =", rf, Pf, f"{mf:.3f} ms; bwd rb,P =", rbk, Pb, f"{mb:.3f} ms")
+    assert rf in (128, 256, 512) and Pf in (4, 8)
+    assert rbk in (128, 256, 512) and Pb in (4, 8)
+    cfg = ctx.batched_config(A, 16)
+    assert cfg["forward"]["rows_per_block"] == rf and cfg["back_projection"]["frames_per_lane"] == Pb
+    ctx_g = env_ctx(monkeypatch, EPET_BATCHED="gather")
+    G = Matrix.from_csr(ctx, rp, col, val, prob.A.n).attach_transpose()
+    (Rf, Pgf, Rb, Pgb), _ = ctx_g.mlem_tune(G, 16)
+    assert Rf in (1, 2, 4, 8) and Pgf in (4, 8, 16, 32) and Pgf <= 64 // Rf
+    assert Rb in (1, 2, 4, 8) and Pgb in (4, 8, 16, 32) and Pgb <= 64 // Rb
     Y = torch.rand(A.m, 16, device="cuda") * 5
     X1 = torch.full((A.n, 16), 1.0 / A.n, device="cuda")
     X2 = X1.clone()
