@@ -79,6 +79,7 @@ SIGNATURES = {
     "epet_mlem": (ctypes.c_int, [P, P, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, c_double_p, c_int_p]),
     "epet_getChords": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, P, P, P]),
+    "epet_mlem_noalloc": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, c_double_p]),
     "epet_mlem_tune": (ctypes.c_int, [P, P, ctypes.c_int, c_int_p, c_float_p]),
     "epet_mlem_kernels": (ctypes.c_int, [P, P, ctypes.c_int, c_int_p, c_float_p]),
     "epet_nInChunk": (ctypes.c_uint32, [ctypes.c_uint32] * 3),

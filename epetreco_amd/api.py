@@ -303,6 +303,18 @@ class Context:
                               iters.ctypes.data_as(_lib.c_int_p)))
         return ll[:niter], iters
 
+    def mlem_noalloc(self, A: "Matrix", y: torch.Tensor, s: torch.Tensor, x: torch.Tensor,
+                     niter: int):
+        """In-place single-frame MLEM on x that allocates no device memory
+        (epet_mlem_noalloc); returns loglik[niter]."""
+        _check_vec(y, A.m, "y", self.device)
+        _check_vec(s, A.n, "s", self.device)
+        _check_vec(x, A.n, "x", self.device)
+        ll = np.zeros(max(niter, 1), dtype=np.float64)
+        check(lib().epet_mlem_noalloc(self._h, A.handle, _p(y), _p(s), _p(x), niter,
+                                      ll.ctypes.data_as(_lib.c_double_p)))
+        return ll[:niter]
+
 
 class Matrix:
     """Library-owned ECSR matrix (epet_matrix)."""

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "epet_block.hpp"
@@ -32,6 +33,8 @@ int bp_prepare(epet_ctx* ctx, epet_matrix* A);
 bool bp_use_transpose(const epet_ctx* ctx, const epet_matrix* A);
 int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
                         float beta, float* y);
+int matrix_spmv_t_prepared(epet_ctx* ctx, const epet_matrix* A, float alpha, const float* x,
+                           float beta, float* y);
 int lds_prepare(epet_ctx* ctx, epet_matrix* A);
 int lds_prepare_map(epet_ctx* ctx, epet_matrix* A, bool voxel_columns);
 bool lds_ready(const epet_ctx* ctx, const epet_matrix* A);
@@ -340,10 +343,12 @@ static int launch_fwd_ratio(hipStream_t st, int ncu, const epet_matrix* A, int F
   return EPET_OK;
 }
 
-// One MLEM iteration on `st` (capturable).
+// One MLEM iteration on `st` (capturable).  prepared_only: the single-frame
+// back projection uses the plan A already holds and builds none (no
+// allocation).
 static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const float* y,
                           const float* s, float* x, float* e, float* c, int* itp, double* sum,
-                          double* loglik, const int* active) {
+                          double* loglik, const int* active, bool prepared_only = false) {
   hipStream_t st = ctx->stream;
   int rc;
   if (F > 1 && A->tr) {
@@ -362,6 +367,7 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
     }
     if (rc != EPET_OK) return rc;
     if (via_tr) rc = matrix_bp_transpose(ctx, A->tr, 1.0f, e, 0.0f, c);
+    else if (F == 1 && prepared_only) rc = matrix_spmv_t_prepared(ctx, A, 1.0f, e, 1.0f, c);
     else if (F == 1) rc = matrix_spmv(ctx, A, EPET_OP_T, 1.0f, e, 1.0f, c);
     else rc = matrix_spmm(ctx, A, EPET_OP_T, F, 1.0f, e, 1.0f, c);
   }
@@ -373,6 +379,57 @@ static int mlem_iteration(epet_ctx* ctx, const epet_matrix* A, int F, const floa
   k_advance<<<1, 64, 0, st>>>(itp);
   EPET_HIP(hipGetLastError());
   return EPET_OK;
+}
+
+// ------------------------------------------------------ allocation-free MLEM
+// Working memory of epet_mlem_noalloc: one copy per GPU, held until the
+// process ends.  In host code the array's name is only a stand-in; its device
+// address on each GPU comes from hipGetSymbolAddress, taken once per device.
+static __device__ __align__(256) unsigned char scratch[EPET_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (looked up on the first call for
+// that device, with the device current).
+static int scratch_address(int device, unsigned char** out) {
+  EPET_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, EPET_ERR_UNSUPPORTED,
+             "device index beyond " + std::to_string(NOALLOC_MAX_DEVICES));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    int cur = -1;
+    EPET_HIP(hipGetDevice(&cur));
+    if (cur != device) EPET_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || p == nullptr)
+      return fail(EPET_ERR_HIP, __FILE__, __LINE__,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return EPET_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of the working arrays of epet_mlem_noalloc inside `scratch`:
+// doubles {sum of s, x sums [2], log-likelihoods [niter]}, the iteration
+// counter, e (m floats), c (n floats), normalised s (n floats).
+struct NoallocLayout {
+  size_t d, it, e, c, s, total;
+};
+static NoallocLayout noalloc_layout(long long m, long long n, int niter) {
+  NoallocLayout L;
+  L.d = 0;
+  L.it = align256(sizeof(double) * (3 + (size_t)std::max(1, niter)));
+  L.e = L.it + 256;
+  L.c = align256(L.e + sizeof(float) * (size_t)std::max(1ll, m));
+  L.s = align256(L.c + sizeof(float) * (size_t)std::max(1ll, n));
+  L.total = align256(L.s + sizeof(float) * (size_t)std::max(1ll, n));
+  return L;
 }
 
 }  // namespace epet
@@ -562,6 +619,49 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const 
   cleanup();
 #undef EPET_MLEM_HIP
   return rc;
+}
+
+int epet_mlem_noalloc(epet_ctx* ctx, const epet_matrix* A, const float* y, const float* s_in,
+                      float* x, int niter, double* loglik_out) {
+  EPET_CHECK(ctx && A && y && s_in && x, EPET_ERR_ARG, "null argument");
+  EPET_CHECK(niter >= 0, EPET_ERR_ARG, "negative iteration count");
+  const NoallocLayout L = noalloc_layout(A->m, A->n, niter);
+  EPET_CHECK(L.total <= (size_t)EPET_NOALLOC_SCRATCH_BYTES, EPET_ERR_CAPACITY,
+             "allocation-free MLEM needs " + std::to_string(L.total) +
+                 " bytes of working memory (m + 2 n floats, niter doubles) for m = " +
+                 std::to_string(A->m) + ", n = " + std::to_string(A->n) + ", niter = " +
+                 std::to_string(niter) + "; the limit is " +
+                 std::to_string((size_t)EPET_NOALLOC_SCRATCH_BYTES));
+  unsigned char* base = nullptr;
+  EPET_TRY(scratch_address(ctx->device, &base));
+  hipStream_t st = ctx->stream;
+  double* dbuf = (double*)(base + L.d);
+  double* ssum = dbuf;
+  double* sum = dbuf + 1;
+  double* ll = dbuf + 3;
+  int* itp = (int*)(base + L.it);
+  float* e = (float*)(base + L.e);
+  float* c = (float*)(base + L.c);
+  float* s = (float*)(base + L.s);
+  const int n = A->n;
+  // the array keeps the previous call's contents: clear with kernels
+  const int nd = 3 + std::max(1, niter);
+  k_zero_f64<<<grid_for(nd, 256), 256, 0, st>>>(dbuf, nd);
+  k_fill_i32_one<<<1, 64, 0, st>>>(itp, 0);
+  k_fill_f32<<<grid_for(std::max(1, n), 256, 1ll << 31), 256, 0, st>>>(c, n, 0.0f);
+  // s <- s / sum(s)  (reco.cu:182-183)
+  if (n > 0) {
+    k_sum_f64<<<grid_for(n, 256, 2048), 256, 0, st>>>(s_in, n, ssum);
+    k_scale_by_sum<<<grid_for(n, 256, 1ll << 31), 256, 0, st>>>(s, s_in, n, ssum);
+  }
+  EPET_HIP(hipGetLastError());
+  for (int it = 0; it < niter; it++)
+    EPET_TRY(mlem_iteration(ctx, A, 1, y, s, x, e, c, itp, sum, ll, nullptr, true));
+  if (loglik_out && niter > 0) {
+    EPET_HIP(hipMemcpyAsync(loglik_out, ll, sizeof(double) * (size_t)niter, hipMemcpyDeviceToHost, st));
+  }
+  EPET_HIP(hipStreamSynchronize(st));
+  return EPET_OK;
 }
 
 int epet_mlem_tune(epet_ctx* ctx, epet_matrix* A, int F, int* cfg, float* ms) {

@@ -499,6 +499,28 @@ int matrix_spmv(epet_ctx* ctx, const epet_matrix* A, int op, float alpha, const 
   EPET_FAIL(EPET_ERR_STATE, "unknown value type");
 }
 
+// y = alpha A^T x + beta y with the back-projection plan A already holds (the
+// transposed copy's, the tiled plan, or none: the atomic kernel).  Builds no
+// plan, so it allocates nothing.
+template <typename VT>
+static int spmv_t_prepared(epet_ctx* ctx, const epet_matrix* A, float alpha, const float* x,
+                           float beta, float* y) {
+  if (!A->rowscale && bp_use_transpose(ctx, A)) return bp_rows<VT>(ctx, A->tr, alpha, x, beta, y);
+  if (ctx->opt.bp != BP_ATOMIC && A->bp_toff) return launch_bp_tiled<VT>(ctx->stream, A, x, alpha, beta, y);
+  return launch_spmv<VT, int64_t>(ctx->stream, EPET_OP_T, A->m, A->n, A->nnz, A->rowptr, A->col,
+                                  A->val, A->rowscale, x, alpha, beta, y, A->lanes_fwd);
+}
+
+int matrix_spmv_t_prepared(epet_ctx* ctx, const epet_matrix* A, float alpha, const float* x,
+                           float beta, float* y) {
+  switch (A->value_type) {
+    case EPET_VAL_F32: return spmv_t_prepared<ValF32>(ctx, A, alpha, x, beta, y);
+    case EPET_VAL_BF16: return spmv_t_prepared<ValBF16>(ctx, A, alpha, x, beta, y);
+    case EPET_VAL_F16: return spmv_t_prepared<ValF16>(ctx, A, alpha, x, beta, y);
+  }
+  EPET_FAIL(EPET_ERR_STATE, "unknown value type");
+}
+
 int matrix_bp_transpose(epet_ctx* ctx, const epet_matrix* At, float alpha, const float* x,
                         float beta, float* y) {
   switch (At->value_type) {

@@ -244,6 +244,26 @@ int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int nframes, const float* y_d
               const float* s_dev, float* x_dev, int niter, int use_graph,
               double tol, double* loglik_out, int* frame_iters);
 
+/* Single-frame MLEM that allocates nothing: the same iteration as epet_mlem
+ * with nframes = 1, use_graph = 0, tol = 0, run eagerly on the context's
+ * stream, with its working arrays (the ratio e: m floats; the back projection
+ * c and the normalised s: n floats each; niter FP64 log-likelihoods) in a
+ * fixed device array of EPET_NOALLOC_SCRATCH_BYTES per GPU that the library
+ * holds until the process ends.  It makes no hipMalloc/hipFree call and
+ * builds no product plan: it runs the plans A already holds (those that an
+ * earlier epet_mlem call on A built) and otherwise the plan-free kernels.
+ * Sizes whose working arrays exceed the limit fail with EPET_ERR_CAPACITY
+ * before any work.  Calls share their GPU's array and each clears what it
+ * needs: calls that overlap on one GPU (two host threads, or two streams)
+ * race.  loglik_out (host, niter doubles) may be NULL.  Returns after the
+ * stream has finished the iterations (it synchronises, so it is not for use
+ * under stream capture).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid allocation). */
+#define EPET_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int epet_mlem_noalloc(epet_ctx* ctx, const epet_matrix* A, const float* y_dev,
+                      const float* s_dev, float* x_dev, int niter, double* loglik_out);
+
 /* Autotuner of the multi-frame products epet_mlem runs for F frames on A and
  * its attached transpose.  Blocked products (the default): every shape of
  * (rows per block rb in {128,256,512}) x (frames per lane P in {4,8}) that
