@@ -128,6 +128,31 @@ def spmv_bytes(nnz, m, V):
     return nnz * 8 + (m + 1) * 8 + m * 4 + V * 4
 
 
+def batched_bytes(nnz, m, V, F, sval, scaled):
+    """Algorithmic bytes of one F-frame MLEM iteration with sval-byte stored
+    values: both products stream the values and int32 columns once, the int64
+    row pointers and (reduced precision) the row scales; per frame the data,
+    ratio and projections on the row side (12 B per row) and the image,
+    sensitivity, back projection and update on the voxel side (24 B per
+    voxel)."""
+    return 2 * (nnz * (sval + 4) + (m * 4 if scaled else 0) + (m + 1) * 8) + F * (12 * m + 24 * V)
+
+
+def batched_traffic(workload, storage):
+    """HBM bytes per batched iteration of the blocked products (both
+    directions, the split reduction and the gather) from the committed counter
+    summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_batched_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("storage", {}).get(storage)
+    if k is None:
+        return None
+    return k["fetch_bytes"] * t["copy_scale_fetch"] + k["write_bytes"] * t["copy_scale_write"]
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter's reading
@@ -221,8 +246,14 @@ def main():
         say(f"tuned in {time.perf_counter() - t0:.2f} s (incl. blocked copies): {cfg} {tms}")
         ctx.mlem(M, Yp, prob.s, Xp, 3, use_graph=False)
         torch.cuda.synchronize()
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)       # elementwise kernel: 1 GiB read, 1 GiB written
+        torch.cuda.synchronize()
         print(json.dumps({"profile_batched": args.profile_batched, "frames": F,
-                          "config": ctx.batched_config(M, F)}))
+                          "config": ctx.batched_config(M, F), "iterations": 3,
+                          "copy_bytes_each_way": src.numel() * 4}))
         return
 
     if args.profile_kernels:
@@ -293,7 +324,7 @@ def main():
         pick = sorted({int(np.argmin(totals)), int(np.argmax(totals)), 0, F - 1})
         acc_iters = args.accuracy_iters
 
-        def run_batched(M, label):
+        def run_batched(M, label, key, sval, scaled):
             t0 = time.perf_counter()
             M.attach_transpose()
             torch.cuda.synchronize()
@@ -309,9 +340,20 @@ def main():
             fis = args.batched_steps * F / t_b
             say(f"  {label} x{F}: {1e3 * t_b / args.batched_steps:.2f} ms/iteration, "
                 f"{fis:.1f} frame-iter/s ({fis / plain_fis:.2f}x plain FP32); transpose {t_tr:.2f} s")
-            res = {"frame_iter_per_s": fis, "ms_per_iteration": 1e3 * t_b / args.batched_steps,
+            ms_it = 1e3 * t_b / args.batched_steps
+            B_it = batched_bytes(M.nnz, m, V, F, sval, scaled)
+            tr = batched_traffic(args.workload, key)
+            res = {"frame_iter_per_s": fis, "ms_per_iteration": ms_it,
                    "speedup_vs_plain_fp32": fis / plain_fis, "iterations_timed": args.batched_steps,
-                   "kernels": ctx.batched_config(M, F)}
+                   "kernels": ctx.batched_config(M, F),
+                   "roofline": {"bound": "hbm", "achieved": B_it / ms_it / 1e6,
+                                "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                                "frac": B_it / ms_it / 1e6 / HBM_PEAK_GBPS,
+                                "algorithmic_bytes": B_it, "traffic": tr,
+                                "scope": "one F-frame MLEM iteration (graph replay, all kernels)"}}
+            say(f"    {B_it / 1e9:.1f} GB algorithmic per iteration: {B_it / ms_it / 1e6:.0f} GB/s, "
+                f"{B_it / ms_it / 1e6 / HBM_PEAK_GBPS:.1%} of 8 TB/s"
+                + (f"; counters {tr / 1e9:.1f} GB" if tr else ""))
             Xa = None
             if acc_iters > 0:
                 Xa = torch.full((V, len(pick)), 1.0 / V, device="cuda")
@@ -324,12 +366,12 @@ def main():
         for key, vt, label in (("bf16_row_scale", VAL_BF16, "BF16 storage"),
                                ("fp16_row_scale", VAL_F16, "FP16 storage")):
             M = A.convert(vt)
-            runs[key] = run_batched(M, label)
+            runs[key] = run_batched(M, label, key, 2, True)
             M.close()
             del M
             gc.collect()
             torch.cuda.empty_cache()
-        runs["fp32"] = run_batched(A, "FP32 storage")
+        runs["fp32"] = run_batched(A, "FP32 storage", "fp32", 4, False)
         Xf = runs["fp32"][0]
         batched = {"frames": F}
         for key, (Xk, rk, _) in runs.items():

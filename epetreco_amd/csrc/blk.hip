@@ -7,14 +7,18 @@
 //   * rows in a locality order (rowperm: position -> row) and columns in a
 //     locality order (colinv: position -> column), both cut into pieces: row
 //     blocks of RB positions (chosen at launch, a multiple of BLK_G) and column
-//     tiles of C positions (C Fp floats = 128 KB of LDS, Fp = padded frames);
+//     tiles of C positions (C Fp floats = 64 KB of LDS, Fp = padded frames);
 //   * entries tile-major: for each tile, the segments of all row positions back
 //     to back, each padded with zero entries to a multiple of 4 (16-byte loads);
 //     an entry is (tile-local column, stored value), packed in 32 bits for
 //     16-bit values;
 //   * cnt[h m_pad + p]: padded length of row position p's segment in tile h;
 //     seg_base[h (nG+1) + j]: first entry of granule j (BLK_G row positions) in
-//     tile h, so that a row block's entries in a tile are one contiguous range.
+//     tile h, so that a row block's entries in a tile are one contiguous range;
+//     nz_base[h (nG+1) + j]: row positions with entries in tile h before
+//     granule j (a running count over tiles), so that a wavefront places its
+//     granule's rows in the block's compacted row list without a block-wide
+//     scan.
 //
 // A workgroup owns a row block and keeps its rows' F-frame sums in LDS while
 // it walks the column tiles: for every tile the block touches it stages the
@@ -23,13 +27,17 @@
 // each slot walks a contiguous run of entry quads, accumulates in registers
 // and adds a row's partial sum into the LDS row sums (LDS float atomics) when
 // the row or the run ends.  Only a quarter of a block's rows meet a given tile,
-// so entries, not rows, are what is balanced.  The matrix streams once, no
+// so entries, not rows, are what is balanced.  Loads are requested a step
+// ahead: the next tile's slice of G, counts and bases while the current tile
+// is worked, and each slot's next batch of entry quads (within the run, or
+// the first of its run in the next tile) while the current batch is.  The
+// matrix streams once, no
 // partial sums per (row, tile) reach memory, and the complete row sums feed
 // the epilogue: the MLEM ratio and the FP64 log-likelihood for the forward
 // projection, the image for the back projection.  When a direction has few
-// row blocks (the back projection's voxel rows), each block's tiles are split
-// by entry count over `splits` workgroups that write partial sums, which a
-// reduction adds in split order.  The grid is persistent (one workgroup per CU
+// row blocks (the back projection's voxel rows), each block's tile range is
+// cut into `splits` equal parts for as many workgroups, which write partial
+// sums that a reduction adds in split order.  The grid is persistent (one workgroup per CU
 // looping over the work items), and the (rows per block, frames per lane)
 // shape is autotuned per matrix and frame count.
 #include <hip/hip_bf16.h>
@@ -51,11 +59,15 @@ constexpr int BLK_THREADS = 1024;
 constexpr int BLK_WAVES = BLK_THREADS / WAVE;
 constexpr int BLK_TILE_FLOATS = 16384;      // 64 KB of the gathered vector per tile
 constexpr int BLK_G = 64;                   // row granule of the segment bases
-constexpr int BLK_MAX_RB = BLK_THREADS;     // rows per block (one row per thread in the scan)
+constexpr int BLK_MAX_RB = BLK_THREADS;     // rows per block (one row per thread)
 constexpr int BLK_LDS_LIMIT = 160 * 1024 - 2048;   // dynamic LDS per workgroup (gfx950)
-// entry quads in flight per slot (fewer with 8 frames per lane, whose
-// gathered rows take twice the registers)
-template <int P> constexpr int blk_u() { return P == 4 ? 4 : 2; }
+// entry quads per batch of a slot; a lane holds two batches (the one it works
+// on and the next, requested ahead), so fewer with 8 frames per lane, whose
+// gathered rows take twice the registers, and with FP32 values, whose
+// entries take 6 bytes
+template <typename VT, int P> constexpr int blk_u() {
+  return sizeof(typename VT::T) == 4 ? (P == 4 ? 2 : 1) : (P == 4 ? 4 : 2);
+}
 
 // Blocked copy of one matrix for one padded frame count.
 struct BlkCopy {
@@ -66,6 +78,7 @@ struct BlkCopy {
   int* rowperm = nullptr;      // m_pad: position -> row, -1 past m
   int* colinv = nullptr;       // T C: position -> column, -1 past n
   int64_t* seg_base = nullptr; // T (nG + 1)
+  int64_t* nz_base = nullptr;  // T (nG + 1)
   uint16_t* cnt = nullptr;     // T m_pad
   void* ent = nullptr;         // u32 packed (16-bit values) or u16 columns (FP32)
   float* eval = nullptr;       // FP32 values (FP32 storage)
@@ -73,6 +86,7 @@ struct BlkCopy {
   float* partial = nullptr;    // splits x m_pad x Fp
   long long partial_len = 0;
   float* gp = nullptr;         // the gathered vector in column-position order, T C x Fp
+  int* work = nullptr;         // {next work item, finished workgroups}, zero between launches
   // launch shape (autotuned): frames per lane, rows per block, splits
   int P = 4, rb = 256, splits = 1;
   int tuned_F = 0;
@@ -82,11 +96,13 @@ struct BlkCopy {
     if (rowperm) (void)hipFree(rowperm);
     if (colinv) (void)hipFree(colinv);
     if (seg_base) (void)hipFree(seg_base);
+    if (nz_base) (void)hipFree(nz_base);
     if (cnt) (void)hipFree(cnt);
     if (ent) (void)hipFree(ent);
     if (eval) (void)hipFree(eval);
     if (partial) (void)hipFree(partial);
     if (gp) (void)hipFree(gp);
+    if (work) (void)hipFree(work);
     delete next;
   }
 };
@@ -104,6 +120,7 @@ struct BlkArgs {
   int mode;                    // 0: forward MLEM ratio, 1: image, 2: partial sums
   const int* rowperm;
   const int64_t* seg_base;
+  const int64_t* nz_base;
   const uint16_t* cnt;
   const void* ent;
   const float* eval;
@@ -112,6 +129,7 @@ struct BlkArgs {
   float* out;                  // forward: ratio in position order (m_pad x Fp);
                                // back projection: image (n x F, frame-minor)
   float* partial;
+  int* work;                   // work-item counter and finished-workgroup count
   const float* rowscale;       // forward: per-row scale of the stored values
   int fold;                    // forward: the ratio carries the row scale
   const float* Y;              // forward: data (m x F)
@@ -120,14 +138,17 @@ struct BlkArgs {
 };
 
 // ------------------------------------------------------------------ entries
-// Four consecutive entries (16-byte aligned): the columns as byte offsets of
-// their frames in the 64 KB LDS tile (the 16-bit mask bounds every LDS address)
-// and the values.
+// Four consecutive entries (16-byte aligned), fetched raw and decoded when
+// used: the columns as byte offsets of their frames in the 64 KB LDS tile (the
+// 16-bit mask bounds every LDS address) and the values.
 template <typename VT> struct BlkEnt;
 template <> struct BlkEnt<ValBF16> {
-  static __device__ __forceinline__ void load(const BlkArgs& a, long long e, unsigned cm,
-                                              unsigned c[4], float v[4]) {
-    const uint4 pk = *reinterpret_cast<const uint4*>(static_cast<const uint32_t*>(a.ent) + e);
+  typedef uint4 Raw;
+  static __device__ __forceinline__ Raw fetch(const BlkArgs& a, long long e) {
+    return *reinterpret_cast<const uint4*>(static_cast<const uint32_t*>(a.ent) + e);
+  }
+  static __device__ __forceinline__ void decode(const Raw& pk, unsigned cm, unsigned c[4],
+                                                float v[4]) {
     c[0] = pk.x & cm; c[1] = pk.y & cm; c[2] = pk.z & cm; c[3] = pk.w & cm;
     v[0] = __uint_as_float(pk.x & 0xFFFF0000u);
     v[1] = __uint_as_float(pk.y & 0xFFFF0000u);
@@ -136,9 +157,12 @@ template <> struct BlkEnt<ValBF16> {
   }
 };
 template <> struct BlkEnt<ValF16> {
-  static __device__ __forceinline__ void load(const BlkArgs& a, long long e, unsigned cm,
-                                              unsigned c[4], float v[4]) {
-    const uint4 pk = *reinterpret_cast<const uint4*>(static_cast<const uint32_t*>(a.ent) + e);
+  typedef uint4 Raw;
+  static __device__ __forceinline__ Raw fetch(const BlkArgs& a, long long e) {
+    return *reinterpret_cast<const uint4*>(static_cast<const uint32_t*>(a.ent) + e);
+  }
+  static __device__ __forceinline__ void decode(const Raw& pk, unsigned cm, unsigned c[4],
+                                                float v[4]) {
     c[0] = pk.x & cm; c[1] = pk.y & cm; c[2] = pk.z & cm; c[3] = pk.w & cm;
     v[0] = __half2float(__ushort_as_half((uint16_t)(pk.x >> 16)));
     v[1] = __half2float(__ushort_as_half((uint16_t)(pk.y >> 16)));
@@ -146,14 +170,23 @@ template <> struct BlkEnt<ValF16> {
     v[3] = __half2float(__ushort_as_half((uint16_t)(pk.w >> 16)));
   }
 };
+typedef unsigned short blk_us4 __attribute__((ext_vector_type(4)));
+struct BlkRawF32 {
+  blk_us4 cc;
+  float4 vv;
+};
 template <> struct BlkEnt<ValF32> {
-  static __device__ __forceinline__ void load(const BlkArgs& a, long long e, unsigned cm,
-                                              unsigned c[4], float v[4]) {
-    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
-    const us4 cc = *reinterpret_cast<const us4*>(static_cast<const uint16_t*>(a.ent) + e);
-    const float4 vv = *reinterpret_cast<const float4*>(a.eval + e);
-    c[0] = cc.x & cm; c[1] = cc.y & cm; c[2] = cc.z & cm; c[3] = cc.w & cm;
-    v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
+  typedef BlkRawF32 Raw;
+  static __device__ __forceinline__ Raw fetch(const BlkArgs& a, long long e) {
+    Raw r;
+    r.cc = *reinterpret_cast<const blk_us4*>(static_cast<const uint16_t*>(a.ent) + e);
+    r.vv = *reinterpret_cast<const float4*>(a.eval + e);
+    return r;
+  }
+  static __device__ __forceinline__ void decode(const Raw& r, unsigned cm, unsigned c[4],
+                                                float v[4]) {
+    c[0] = r.cc.x & cm; c[1] = r.cc.y & cm; c[2] = r.cc.z & cm; c[3] = r.cc.w & cm;
+    v[0] = r.vv.x; v[1] = r.vv.y; v[2] = r.vv.z; v[3] = r.vv.w;
   }
 };
 
@@ -162,14 +195,20 @@ template <> struct BlkEnt<ValF32> {
 // address 0 and an entry's byte offset is its LDS address):
 //   tile      C Fp floats (64 KB) of Gp
 //   acc       rb Fp floats: the block's row sums
-//   nz_row    rb ints, nz_start rb + 1 ints: rows with entries in the tile
-//   tl_c0     BLK_THREADS int64, tl_h BLK_THREADS ints: non-empty tiles of a
-//             chunk of BLK_THREADS tiles and the block's first entry in each
-//   wsum      2 x 16 ints, cnt 1 int; ll 64 doubles
-constexpr int BLK_LDS_TAIL = sizeof(int64_t) * BLK_THREADS + sizeof(int) * BLK_THREADS +
-                             sizeof(int) * (2 * BLK_WAVES + 2) + sizeof(double) * 64;
+//   nz_row    rb ints, nz_end rb ints: the block's rows with entries in the
+//             tile (compacted, in position order) and their segment ends
+//   slot_k    BLK_MAX_SLOTS ints: the compacted row holding each slot's first
+//             entry
+//   tl_c0     BLK_THREADS int64, tl_h and tl_n BLK_THREADS ints: non-empty
+//             tiles of a chunk of BLK_THREADS tiles, the block's first entry
+//             and its entry count in each
+//   wsum      2 x 16 + 2 ints; ll 64 doubles
+constexpr int BLK_MAX_SLOTS = BLK_WAVES * WAVE;   // one lane per slot at most
+constexpr int BLK_LDS_TAIL = sizeof(int) * BLK_MAX_SLOTS + sizeof(int64_t) * BLK_THREADS +
+                             2 * sizeof(int) * BLK_THREADS + sizeof(int) * (2 * BLK_WAVES + 2) +
+                             sizeof(double) * 64;
 __host__ __device__ inline size_t blk_lds_bytes(int C, int Fp, int rb) {
-  return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) + sizeof(int) * (2 * (size_t)rb + 2) +
+  return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) + sizeof(int) * (2 * (size_t)rb) +
          BLK_LDS_TAIL;
 }
 
@@ -202,85 +241,141 @@ __device__ __forceinline__ void blk_scan2(int xa, int xb, int* wsum, int* pa, in
   *tb = sb;
 }
 
+// Inclusive scan over a wavefront.
+__device__ __forceinline__ int blk_wave_scan(int x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const int z = __shfl_up(x, off, WAVE);
+    if (lane >= off) x += z;
+  }
+  return x;
+}
+
+// A slot's run in a tile of tc entries: an equal share of the tile's entry
+// quads (NS slots, a power of two; nq NS < 2^31 since tc <= rb C < 2^21).
+__device__ __forceinline__ void blk_run(int tc, int ks, int lgNS, int* e0, int* e1) {
+  const int nq = tc >> 2;
+  *e0 = ((nq * ks) >> lgNS) << 2;
+  *e1 = ((nq * (ks + 1)) >> lgNS) << 2;
+}
+
 template <typename VT, int P>
 __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   extern __shared__ float4 blk_lds[];
+  typedef typename BlkEnt<VT>::Raw Raw;
+  constexpr int U = blk_u<VT, P>();             // entry quads per batch
   float* xs = reinterpret_cast<float*>(blk_lds);
   float* acc = xs + ((size_t)a.C << a.lgFp);
   int* nz_row = reinterpret_cast<int*>(acc + ((size_t)a.rb << a.lgFp));
-  int* nz_start = nz_row + a.rb;
-  int64_t* tl_c0 = reinterpret_cast<int64_t*>(nz_start + a.rb + 2);
+  int* nz_end = nz_row + a.rb;
+  int* slot_k = nz_end + a.rb;
+  int64_t* tl_c0 = reinterpret_cast<int64_t*>(slot_k + BLK_MAX_SLOTS);
   int* tl_h = reinterpret_cast<int*>(tl_c0 + BLK_THREADS);
-  int* wsum = tl_h + BLK_THREADS;
+  int* tl_n = tl_h + BLK_THREADS;
+  int* wsum = tl_n + BLK_THREADS;
   double* s_ll = reinterpret_cast<double*>(wsum + 2 * BLK_WAVES + 2);
   const int t = threadIdx.x;
   const int lane = lane_id();
-  const int w = wave_id();
+  const int w = __builtin_amdgcn_readfirstlane(wave_id());
   const int slot = lane / a.L, q = lane % a.L;
   const int ks = w * a.S + slot;           // this lane's slot
+  const int NS = 1 << a.lgNS;
   const unsigned cm = 0xFFFFu;
   const int rbfp = a.rb << a.lgFp;
   const int tile4 = (a.C << a.lgFp) >> 2;  // float4 per tile (4096)
   const unsigned lane_off = (unsigned)(q * P * 4);
   const char* xb = reinterpret_cast<const char*>(xs);
+  const float4* g4 = reinterpret_cast<const float4*>(a.Gp);
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
   if (t < 64) s_ll[t] = 0.0;
   double ll = 0.0;                         // log-likelihood of frame t mod Fp
-  const long long nwork = (long long)a.nblocks * a.splits;
-  for (long long work = blockIdx.x; work < nwork; work += gridDim.x) {
-    const int blk = (int)(work / a.splits), sp = (int)(work % a.splits);
+  int* s_item = wsum + 2 * BLK_WAVES;     // the work item this workgroup took
+  const int nwork = a.nblocks * a.splits;
+  // work items are taken in order from a counter, so a workgroup that finishes
+  // early takes the next one
+  for (;;) {
+    __syncthreads();                       // the previous item's epilogue is done
+    if (t == 0) *s_item = atomicAdd(a.work, 1);
+    __syncthreads();
+    const int work = *s_item;
+    if (work >= nwork) break;
+    const int blk = work / a.splits, sp = work % a.splits;
     const int p0 = blk * a.rb;
     const int rbe = min(a.rb, a.m_pad - p0);
     const int j0 = p0 / BLK_G, j1 = (p0 + rbe) / BLK_G;
+    const bool wrows = w * WAVE < rbe;     // this wavefront holds rows (granule j0 + w)
     // this work item's tiles: an equal share of the tile range
     const int hlo = (int)((long long)a.T * sp / a.splits);
     const int hhi = (int)((long long)a.T * (sp + 1) / a.splits);
-    __syncthreads();                       // the previous block's epilogue is done
     for (int i = t; i < rbfp; i += BLK_THREADS) acc[i] = 0.0f;
     for (int hb = hlo; hb < hhi; hb += BLK_THREADS) {
       // the block's non-empty tiles among the next BLK_THREADS, in order, with
-      // its first entry in each
+      // its first entry and entry count in each
       const int hh = hb + t;
       long long c0h = 0;
-      int fl = 0;
+      int nh = 0;
       if (hh < hhi) {
         const int64_t* sb = a.seg_base + (long long)hh * (a.nG + 1);
         c0h = sb[j0];
-        fl = sb[j1] > c0h ? 1 : 0;
+        nh = (int)(sb[j1] - c0h);
       }
       int pk, pz, ntl, dz;
       __syncthreads();                     // the previous chunk's list is no longer read
-      blk_scan2(fl, 0, wsum, &pk, &pz, &ntl, &dz);
-      if (fl) {
+      blk_scan2(nh > 0 ? 1 : 0, 0, wsum, &pk, &pz, &ntl, &dz);
+      if (nh > 0) {
         tl_h[pk] = hh;
         tl_c0[pk] = c0h;
+        tl_n[pk] = nh;
       }
       __syncthreads();
       if (ntl == 0) continue;
-      // the first tile's slice of Gp and row counts, requested ahead
-      const float4* g4 = reinterpret_cast<const float4*>(a.Gp);
-      long long gb = (long long)tl_h[0] * tile4 + t;
+      // the first tile's slice of Gp, row counts and granule bases, requested ahead
+      int hcur = tl_h[0];
+      long long gb = (long long)hcur * tile4 + t;
       float4 g0 = g4[gb], g1 = g4[gb + BLK_THREADS], g2 = g4[gb + 2 * BLK_THREADS],
              g3 = g4[gb + 3 * BLK_THREADS];
-      int cn = t < rbe ? (int)a.cnt[(long long)tl_h[0] * a.m_pad + p0 + t] : 0;
+      int cn = t < rbe ? (int)a.cnt[(long long)hcur * a.m_pad + p0 + t] : 0;
+      long long sbw = 0, nzw = 0, nz0 = 0;
+      if (wrows) {
+        const long long o = (long long)hcur * (a.nG + 1);
+        sbw = a.seg_base[o + j0 + w];
+        nzw = a.nz_base[o + j0 + w];
+        nz0 = a.nz_base[o + j0];
+      }
+      // entry batches: bq holds the next batch this lane works on; pf says
+      // that it was requested during the previous tile
+      Raw bq[U];
+      bool pf = false;
       for (int it = 0; it < ntl; it++) {
         const long long c0 = tl_c0[it];
+        const int tc = tl_n[it];
         __syncthreads();                   // the previous tile's readers are done
         float4* x4 = reinterpret_cast<float4*>(xs);
         x4[t] = g0;
         x4[t + BLK_THREADS] = g1;
         x4[t + 2 * BLK_THREADS] = g2;
         x4[t + 3 * BLK_THREADS] = g3;
-        // the compacted list of the block's rows with entries in this tile and
-        // their segment starts
-        int pc, pn, tc, tn;
-        blk_scan2(cn, cn > 0 ? 1 : 0, wsum, &pc, &pn, &tc, &tn);
-        if (cn > 0) {
-          nz_row[pn] = t;
-          nz_start[pn] = pc;
+        // the compacted list of the block's rows with entries in this tile, their
+        // segment ends, and the row holding each slot's first entry
+        if (wrows) {
+          const int pc = (int)(sbw - c0) + blk_wave_scan(cn) - cn;
+          const int pn = (int)(nzw - nz0) + __popcll(__ballot(cn > 0) & lt_mask);
+          if (cn > 0) {
+            nz_row[pn] = t;
+            nz_end[pn] = pc + cn;
+            const unsigned nq = (unsigned)tc >> 2;
+            // first slot whose run starts at or after pc
+            unsigned k = (((unsigned)pc >> 2 << a.lgNS) + nq - 1) / nq;
+            for (; k < (unsigned)NS; k++) {
+              const int es = (int)(((nq * k) >> a.lgNS) << 2);
+              if (es >= pc + cn) break;
+              slot_k[k] = pn;
+            }
+          }
         }
-        if (t == 0) nz_start[tn] = tc;
-        // the next tile's slice and counts are requested before this tile's
-        // work (the last tile requests its own again)
+        // the next tile's slice, counts and bases are requested before this
+        // tile's work (the last tile requests its own again)
         {
           const int hn = tl_h[min(it + 1, ntl - 1)];
           gb = (long long)hn * tile4 + t;
@@ -289,73 +384,99 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
           g2 = g4[gb + 2 * BLK_THREADS];
           g3 = g4[gb + 3 * BLK_THREADS];
           cn = t < rbe ? (int)a.cnt[(long long)hn * a.m_pad + p0 + t] : 0;
+          if (wrows) {
+            const long long o = (long long)hn * (a.nG + 1);
+            sbw = a.seg_base[o + j0 + w];
+            nzw = a.nz_base[o + j0 + w];
+            nz0 = a.nz_base[o + j0];
+          }
+        }
+        int e, e_end;
+        blk_run(tc, ks, a.lgNS, &e, &e_end);
+        // the next tile's run start, for the batch requested ahead
+        long long c0n = 0;
+        int en = 0, en_end = 0;
+        const bool has_next = it + 1 < ntl;
+        if (has_next) {
+          c0n = tl_c0[it + 1];
+          blk_run(tl_n[it + 1], ks, a.lgNS, &en, &en_end);
+        }
+        if (e < e_end && !pf) {
+#pragma unroll
+          for (int u = 0; u < U; u++)
+            bq[u] = BlkEnt<VT>::fetch(a, e + 4 * u < e_end ? c0 + e + 4 * u : a.zero_quad);
+        }
+        pf = false;
+        if (e >= e_end && has_next) {
+          // nothing to do here: request the next tile's first batch now
+#pragma unroll
+          for (int u = 0; u < U; u++)
+            bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
+          pf = true;
         }
         __syncthreads();
-        // this slot's run of entry quads: an equal share of the tile's entries
-        // (NS slots, a power of two; nq NS < 2^31 for tiles of 64 KB)
-        const int nq = tc >> 2;
-        int e = ((nq * ks) >> a.lgNS) << 2;
-        const int e_end = ((nq * (ks + 1)) >> a.lgNS) << 2;
-        int kk = 0;
-        if (e < e_end) {
-          // the compacted row holding entry e: the last one starting at or before e
-          int lo = 0, hi = tn - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (nz_start[mid] <= e) lo = mid; else hi = mid - 1;
-          }
-          kk = lo;
-        }
-        int row = nz_row[min(kk, max(tn - 1, 0))];
-        int rend = nz_start[min(kk + 1, tn)];
+        int kk = e < e_end ? slot_k[ks] : 0;
+        int row = nz_row[kk];
+        int rend = nz_end[kk];
         float a4[P];
 #pragma unroll
         for (int j = 0; j < P; j++) a4[j] = 0.0f;
         for (;;) {
           if (!__any(e < e_end)) break;
-          // BLK_U entry quads of the run requested at once; quads past the run
-          // read the zero quad stored after the entries
-          constexpr int BLK_U = blk_u<P>();
-          unsigned c[BLK_U][4];
-          float v[BLK_U][4];
+          if (e < e_end) {
+            Raw cur[U];
 #pragma unroll
-          for (int u = 0; u < BLK_U; u++) {
-            const long long eu = e + 4 * u < e_end ? c0 + e + 4 * u : a.zero_quad;
-            BlkEnt<VT>::load(a, eu, cm, c[u], v[u]);
-          }
+            for (int u = 0; u < U; u++) cur[u] = bq[u];
+            // the following batch: the run's next one, else the next tile's first
+            if (e + 4 * U < e_end) {
 #pragma unroll
-          for (int u = 0; u < BLK_U; u++) {
-            // the quad's four gathered rows requested together, then accumulated
-            float4 xv[4][P / 4];
+              for (int u = 0; u < U; u++) {
+                const int eu = e + 4 * (U + u);
+                bq[u] = BlkEnt<VT>::fetch(a, eu < e_end ? c0 + eu : a.zero_quad);
+              }
+            } else if (has_next) {
 #pragma unroll
-            for (int t4 = 0; t4 < 4; t4++) {
-              const float4* xp = reinterpret_cast<const float4*>(xb + (c[u][t4] | lane_off));
-#pragma unroll
-              for (int i = 0; i < P / 4; i++) xv[t4][i] = xp[i];
+              for (int u = 0; u < U; u++)
+                bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
+              pf = true;
             }
 #pragma unroll
-            for (int t4 = 0; t4 < 4; t4++)
+            for (int u = 0; u < U; u++) {
+              unsigned c[4];
+              float v[4];
+              BlkEnt<VT>::decode(cur[u], cm, c, v);
+              // the quad's four gathered rows requested together, then accumulated
+              float4 xv[4][P / 4];
 #pragma unroll
-              for (int i = 0; i < P / 4; i++) {
-                a4[4 * i + 0] += v[u][t4] * xv[t4][i].x;
-                a4[4 * i + 1] += v[u][t4] * xv[t4][i].y;
-                a4[4 * i + 2] += v[u][t4] * xv[t4][i].z;
-                a4[4 * i + 3] += v[u][t4] * xv[t4][i].w;
+              for (int t4 = 0; t4 < 4; t4++) {
+                const float4* xp = reinterpret_cast<const float4*>(xb + (c[t4] | lane_off));
+#pragma unroll
+                for (int i = 0; i < P / 4; i++) xv[t4][i] = xp[i];
               }
-            if (e < e_end) {
-              e += 4;
-              if (e == rend || e == e_end) {
-                // the row (or this slot's run) ends: add its partial sums
-                float* dst = acc + ((size_t)row << a.lgFp) + q * P;
 #pragma unroll
-                for (int j = 0; j < P; j++) {
-                  atomicAdd(dst + j, a4[j]);
-                  a4[j] = 0.0f;
+              for (int t4 = 0; t4 < 4; t4++)
+#pragma unroll
+                for (int i = 0; i < P / 4; i++) {
+                  a4[4 * i + 0] += v[t4] * xv[t4][i].x;
+                  a4[4 * i + 1] += v[t4] * xv[t4][i].y;
+                  a4[4 * i + 2] += v[t4] * xv[t4][i].z;
+                  a4[4 * i + 3] += v[t4] * xv[t4][i].w;
                 }
-                if (e == rend && e < e_end) {
-                  kk++;
-                  row = nz_row[kk];
-                  rend = nz_start[kk + 1];
+              if (e < e_end) {
+                e += 4;
+                if (e == rend || e == e_end) {
+                  // the row (or this slot's run) ends: add its partial sums
+                  float* dst = acc + ((size_t)row << a.lgFp) + q * P;
+#pragma unroll
+                  for (int j = 0; j < P; j++) {
+                    atomicAdd(dst + j, a4[j]);
+                    a4[j] = 0.0f;
+                  }
+                  if (e == rend && e < e_end) {
+                    kk++;
+                    row = nz_row[kk];
+                    rend = nz_end[kk];
+                  }
                 }
               }
             }
@@ -394,6 +515,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
     atomicAdd(&s_ll[t & (a.Fp - 1)], ll);
     __syncthreads();
     if (t < a.F) unsafeAtomicAdd(&a.loglik[(long long)(*a.itp) * a.F + t], s_ll[t]);
+  }
+  // the last workgroup to finish zeroes the counters for the next launch
+  if (t == 0) {
+    __threadfence();
+    if (atomicAdd(a.work + 1, 1) == (int)gridDim.x - 1) {
+      a.work[0] = 0;
+      a.work[1] = 0;
+    }
   }
 }
 
@@ -566,14 +695,25 @@ __global__ __launch_bounds__(256) void k_blk_narrow(const int* __restrict__ c32,
   if (i < n) c16[i] = (uint16_t)c32[i];
 }
 
-// seg_base[h (nG+1) + j] = off[h m_pad + j BLK_G] (j = nG: the tile's end)
-__global__ __launch_bounds__(256) void k_blk_bases(int T, int nG, int m_pad,
+// sb[h (nG+1) + j] = off[h stride + j step] (j = nG: the tile's end, which is
+// the next tile's start)
+__global__ __launch_bounds__(256) void k_blk_bases(int T, int nG, long long stride, int step,
                                                    const int64_t* __restrict__ off,
                                                    int64_t* __restrict__ sb) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)T * (nG + 1)) return;
   const long long h = i / (nG + 1), j = i % (nG + 1);
-  sb[i] = off[h * m_pad + j * BLK_G];
+  sb[i] = off[h * stride + j * step];
+}
+
+// gnz[h nG + j]: row positions of granule j with entries in tile h (one
+// wavefront per granule)
+__global__ __launch_bounds__(256) void k_blk_granule_nz(long long ngran, const int* __restrict__ c32,
+                                                        int* __restrict__ gnz) {
+  const long long g = (long long)blockIdx.x * (256 / WAVE) + wave_id();
+  if (g >= ngran) return;
+  const unsigned long long b = __ballot(c32[g * BLK_G + lane_id()] > 0);
+  if (lane_id() == 0) gnz[g] = __popcll(b);
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32_v(int* __restrict__ p, long long n, int v) {
@@ -681,9 +821,12 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
   const long long ngp = ((long long)B->T * B->C) << B->lgFp;
   bool ok = hipMalloc((void**)&B->gp, sizeof(float) * (size_t)ngp) == hipSuccess &&
             hipMalloc((void**)&B->rowperm, sizeof(int) * (size_t)B->m_pad) == hipSuccess &&
+            hipMalloc((void**)&B->work, sizeof(int) * 2) == hipSuccess &&
             hipMalloc((void**)&B->colinv, sizeof(int) * ci.size()) == hipSuccess &&
             hipMalloc((void**)&cpos_d, sizeof(int) * cpos.size()) == hipSuccess &&
             hipMalloc((void**)&B->seg_base, sizeof(int64_t) * (size_t)B->T * (B->nG + 1)) ==
+                hipSuccess &&
+            hipMalloc((void**)&B->nz_base, sizeof(int64_t) * (size_t)B->T * (B->nG + 1)) ==
                 hipSuccess &&
             hipMalloc((void**)&B->cnt, sizeof(uint16_t) * (size_t)nseg) == hipSuccess &&
             hipMalloc((void**)&cnt32, sizeof(int) * (size_t)nseg) == hipSuccess &&
@@ -703,6 +846,7 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
   EPET_HIP(hipMemcpyAsync(B->colinv, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, st));
   EPET_HIP(hipMemcpyAsync(cpos_d, cpos.data(), sizeof(int) * cpos.size(), hipMemcpyHostToDevice, st));
   k_fill_i32_v<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(cnt32, nseg, 0);
+  k_fill_i32_v<<<1, 64, 0, st>>>(B->work, 2, 0);
   k_fill_i32_v<<<grid_for(ngp, 256, 1ll << 31), 256, 0, st>>>(reinterpret_cast<int*>(B->gp), ngp, 0);
   const size_t lds = sizeof(int) * (256 / WAVE) * (size_t)B->T;
   int rc = EPET_OK;
@@ -731,9 +875,39 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
     k_set_i64_at<<<1, 64, 0, st>>>(off + nseg, tot);
     k_blk_narrow<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(cnt32, nseg, B->cnt);
     k_blk_bases<<<grid_for((long long)B->T * (B->nG + 1), 256, 1ll << 31), 256, 0, st>>>(
-        B->T, B->nG, B->m_pad, off, B->seg_base);
+        B->T, B->nG, B->m_pad, BLK_G, off, B->seg_base);
+    // row positions with entries per (tile, granule), scanned over tiles and
+    // granules into nz_base
+    const long long ngran = (long long)B->T * B->nG;
+    int* gcount = nullptr;
+    int64_t* nzoff = nullptr;
+    if (hipMalloc((void**)&gcount, sizeof(int) * (size_t)std::max(1ll, ngran)) != hipSuccess ||
+        hipMalloc((void**)&nzoff, sizeof(int64_t) * (size_t)(ngran + 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      if (gcount) (void)hipFree(gcount);
+      cleanup();
+      delete B;
+      return EPET_OK;
+    }
+    k_blk_granule_nz<<<grid_for(ngran, 256 / WAVE, 1ll << 31), 256, 0, st>>>(ngran, cnt32, gcount);
+    long long nzt = 0;
+    rc = exclusive_scan_i32_to_i64(ctx, gcount, nzoff, ngran, &nzt);
+    if (rc == EPET_OK) {
+      k_set_i64_at<<<1, 64, 0, st>>>(nzoff + ngran, nzt);
+      k_blk_bases<<<grid_for((long long)B->T * (B->nG + 1), 256, 1ll << 31), 256, 0, st>>>(
+          B->T, B->nG, B->nG, 1, nzoff, B->nz_base);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "blocked copy row bases");
+    }
+    (void)hipFree(gcount);
+    (void)hipFree(nzoff);
     (void)hipFree(cnt32);
     cnt32 = nullptr;
+    if (rc != EPET_OK) {
+      cleanup();
+      delete B;
+      return rc;
+    }
     // entries, then one zero quad that reads past a run are pointed at
     const size_t eb = M->value_type == EPET_VAL_F32 ? 2 : 4;
     if (hipMalloc(&B->ent, eb * (size_t)(tot + 4)) != hipSuccess ||
@@ -789,10 +963,11 @@ static bool blk_shape_valid(const BlkCopy* B, int P, int rb) {
   return blk_lds_bytes(B->C, B->Fp, rb) <= (size_t)BLK_LDS_LIMIT;
 }
 
-// work-item splits of a direction: enough work items for every CU twice over
+// work-item splits of a direction: eight work items per CU at least, so that
+// the items left when the first workgroups run out of work are small
 static int blk_splits(const BlkCopy* B, int rb, int ncu) {
   const int nblocks = (B->m_pad + rb - 1) / rb;
-  const int want = 2 * ncu;
+  const int want = 8 * ncu;
   if (nblocks >= want) return 1;
   return std::min(B->T, std::max(1, (want + nblocks - 1) / nblocks));
 }
@@ -840,6 +1015,7 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits
   a.mode = splits > 1 ? 2 : mode;
   a.rowperm = B->rowperm;
   a.seg_base = B->seg_base;
+  a.nz_base = B->nz_base;
   a.cnt = B->cnt;
   a.ent = B->ent;
   a.eval = B->eval;
@@ -847,6 +1023,7 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits
   a.Gp = B->gp;
   a.out = out;
   a.partial = B->partial;
+  a.work = B->work;
   a.rowscale = rowscale;
   a.fold = fold;
   a.Y = Y;

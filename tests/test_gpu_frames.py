@@ -19,8 +19,10 @@ per-row FP32 scale.  Two figures per frame:
     oracle's own format cost (FP64 MLEM on the stored operator vs on the FP32
     matrix) to within 10% + 1e-4.  MLEM amplifies value rounding by a factor
     that shrinks with the entries per voxel column: this subset has ~18, so
-    BF16 costs ~1.5e-2 here, where the full 5.14M-channel operator (~2e4 per
-    column) stays near 1e-4.
+    BF16 costs ~1.5e-2 here.  The absolute bar (1e-3 against FP64) is held at
+    the full 52^3 operator (~2e4 entries per column) in test_gpu_full.py,
+    where FP16 storage measures <= 1.1e-4 and BF16 up to 1.2e-3 after 20
+    iterations.
 """
 import numpy as np
 import pytest
