@@ -195,21 +195,25 @@ template <> struct BlkEnt<ValF32> {
 // address 0 and an entry's byte offset is its LDS address):
 //   tile      C Fp floats (64 KB) of Gp
 //   acc       rb Fp floats: the block's row sums
-//   nz_row    rb ints, nz_end rb ints: the block's rows with entries in the
-//             tile (compacted, in position order) and their segment ends
-//   slot_k    BLK_MAX_SLOTS ints: the compacted row holding each slot's first
-//             entry
-//   tl_c0     BLK_THREADS int64, tl_h and tl_n BLK_THREADS ints: non-empty
-//             tiles of a chunk of BLK_THREADS tiles, the block's first entry
-//             and its entry count in each
-//   wsum      2 x 16 + 2 ints; ll 64 doubles
+//   lists     two sets, the current tile's and the next one's, each of
+//             row    rb + 4 ints: the block's rows with entries in the tile,
+//                    compacted, in position order,
+//             start  rb + 4 ints: their segment starts (start[rows] = the
+//                    tile's entry count),
+//             slot   BLK_MAX_SLOTS + 4 ints: each slot's first row (slot[NS]
+//                    = rows);
+//   tl_c0     BLK_TL int64, tl_h and tl_n BLK_TL ints: the non-empty tiles of
+//             a chunk of BLK_TL tiles, the block's first entry and its entry
+//             count in each;
+//   wsum      2 x 16 + 2 ints; ll 64 doubles.
 constexpr int BLK_MAX_SLOTS = BLK_WAVES * WAVE;   // one lane per slot at most
-constexpr int BLK_LDS_TAIL = sizeof(int) * BLK_MAX_SLOTS + sizeof(int64_t) * BLK_THREADS +
-                             2 * sizeof(int) * BLK_THREADS + sizeof(int) * (2 * BLK_WAVES + 2) +
-                             sizeof(double) * 64;
+constexpr int BLK_TL = 512;                        // tiles per chunk of the tile list
+__host__ __device__ inline int blk_list_ints(int rb) { return 2 * (rb + 4) + BLK_MAX_SLOTS + 4; }
+constexpr int BLK_LDS_TAIL = sizeof(int64_t) * BLK_TL + 2 * sizeof(int) * BLK_TL +
+                             sizeof(int) * (2 * BLK_WAVES + 2) + sizeof(double) * 64;
 __host__ __device__ inline size_t blk_lds_bytes(int C, int Fp, int rb) {
-  return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) + sizeof(int) * (2 * (size_t)rb) +
-         BLK_LDS_TAIL;
+  return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) +
+         sizeof(int) * 2 * (size_t)blk_list_ints(rb) + BLK_LDS_TAIL;
 }
 
 // Workgroup-wide exclusive scan of (a, b) pairs in thread order: returns this
@@ -252,12 +256,36 @@ __device__ __forceinline__ int blk_wave_scan(int x) {
   return x;
 }
 
-// A slot's run in a tile of tc entries: an equal share of the tile's entry
-// quads (NS slots, a power of two; nq NS < 2^31 since tc <= rb C < 2^21).
-__device__ __forceinline__ void blk_run(int tc, int ks, int lgNS, int* e0, int* e1) {
-  const int nq = tc >> 2;
-  *e0 = ((nq * ks) >> lgNS) << 2;
-  *e1 = ((nq * (ks + 1)) >> lgNS) << 2;
+// One tile's row list (see the LDS layout above).
+struct BlkList {
+  int* row;
+  int* start;
+  int* slot;
+};
+
+// Places row t (cn entries from pc, compacted index pn) of a tile of tc
+// entries in list L, and marks the slots whose first row it is.  Slot k's
+// share of the tile is the entry quads [es(k), es(k+1)), es(k) = 4
+// floor(k (tc / 4) / NS); it takes the rows that start in its share, so every
+// row is one slot's and no two slots add into one row of the sums.
+__device__ __forceinline__ void blk_place(const BlkList& L, int t, int cn, int pc, int pn, int tc,
+                                          int lgNS) {
+  if (cn <= 0) return;
+  const int NS = 1 << lgNS;
+  L.row[pn] = t;
+  L.start[pn] = pc;
+  const unsigned nq = (unsigned)tc >> 2;
+  // first slot whose share starts at or after pc
+  unsigned k = (((unsigned)pc >> 2 << lgNS) + nq - 1) / nq;
+  for (; k < (unsigned)NS; k++) {
+    const int es = (int)(((nq * k) >> lgNS) << 2);
+    if (es >= pc + cn) break;
+    L.slot[k] = es == pc ? pn : pn + 1;
+  }
+  if (pc + cn == tc) {
+    L.start[pn + 1] = tc;
+    L.slot[NS] = pn + 1;
+  }
 }
 
 template <typename VT, int P>
@@ -267,20 +295,26 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   constexpr int U = blk_u<VT, P>();             // entry quads per batch
   float* xs = reinterpret_cast<float*>(blk_lds);
   float* acc = xs + ((size_t)a.C << a.lgFp);
-  int* nz_row = reinterpret_cast<int*>(acc + ((size_t)a.rb << a.lgFp));
-  int* nz_end = nz_row + a.rb;
-  int* slot_k = nz_end + a.rb;
-  int64_t* tl_c0 = reinterpret_cast<int64_t*>(slot_k + BLK_MAX_SLOTS);
-  int* tl_h = reinterpret_cast<int*>(tl_c0 + BLK_THREADS);
-  int* tl_n = tl_h + BLK_THREADS;
-  int* wsum = tl_n + BLK_THREADS;
+  int* lists = reinterpret_cast<int*>(acc + ((size_t)a.rb << a.lgFp));
+  const int li = blk_list_ints(a.rb);
+  // list set s (0 or 1)
+  auto list_set = [&](int s) {
+    BlkList L;
+    L.row = lists + s * li;
+    L.start = L.row + a.rb + 4;
+    L.slot = L.start + a.rb + 4;
+    return L;
+  };
+  int64_t* tl_c0 = reinterpret_cast<int64_t*>(lists + 2 * li);
+  int* tl_h = reinterpret_cast<int*>(tl_c0 + BLK_TL);
+  int* tl_n = tl_h + BLK_TL;
+  int* wsum = tl_n + BLK_TL;
   double* s_ll = reinterpret_cast<double*>(wsum + 2 * BLK_WAVES + 2);
   const int t = threadIdx.x;
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(wave_id());
   const int slot = lane / a.L, q = lane % a.L;
   const int ks = w * a.S + slot;           // this lane's slot
-  const int NS = 1 << a.lgNS;
   const unsigned cm = 0xFFFFu;
   const int rbfp = a.rb << a.lgFp;
   const int tile4 = (a.C << a.lgFp) >> 2;  // float4 per tile (4096)
@@ -309,19 +343,45 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
     const int hlo = (int)((long long)a.T * sp / a.splits);
     const int hhi = (int)((long long)a.T * (sp + 1) / a.splits);
     for (int i = t; i < rbfp; i += BLK_THREADS) acc[i] = 0.0f;
-    for (int hb = hlo; hb < hhi; hb += BLK_THREADS) {
-      // the block's non-empty tiles among the next BLK_THREADS, in order, with
-      // its first entry and entry count in each
+    // a tile's row counts and granule bases for this block
+    int cn = 0;
+    long long sbw = 0, nzw = 0, nz0 = 0;
+    auto request_meta = [&](int h) {
+      cn = t < rbe ? (int)a.cnt[(long long)h * a.m_pad + p0 + t] : 0;
+      if (wrows) {
+        const long long o = (long long)h * (a.nG + 1);
+        sbw = a.seg_base[o + j0 + w];
+        nzw = a.nz_base[o + j0 + w];
+        nz0 = a.nz_base[o + j0];
+      }
+    };
+    // the row list of a tile (first entry c0x, tc entries) from the requested
+    // counts and bases: a wavefront places its granule's rows
+    auto build = [&](const BlkList& L, long long c0x, int tcx) {
+      if (wrows) {
+        const int pc = (int)(sbw - c0x) + blk_wave_scan(cn) - cn;
+        const int pn = (int)(nzw - nz0) + __popcll(__ballot(cn > 0) & lt_mask);
+        blk_place(L, t, cn, pc, pn, tcx, a.lgNS);
+      }
+    };
+    // this slot's rows and entries in a tile whose list is L
+    auto run_of = [&](const BlkList& L, int* e0, int* e1) {
+      *e0 = L.start[L.slot[ks]];
+      *e1 = L.start[L.slot[ks + 1]];
+    };
+    for (int hb = hlo; hb < hhi; hb += BLK_TL) {
+      // the block's non-empty tiles among the next BLK_TL, in order, with its
+      // first entry and entry count in each
       const int hh = hb + t;
       long long c0h = 0;
       int nh = 0;
-      if (hh < hhi) {
+      if (t < BLK_TL && hh < hhi) {
         const int64_t* sb = a.seg_base + (long long)hh * (a.nG + 1);
         c0h = sb[j0];
         nh = (int)(sb[j1] - c0h);
       }
       int pk, pz, ntl, dz;
-      __syncthreads();                     // the previous chunk's list is no longer read
+      __syncthreads();                     // the previous chunk's lists are no longer read
       blk_scan2(nh > 0 ? 1 : 0, 0, wsum, &pk, &pz, &ntl, &dz);
       if (nh > 0) {
         tl_h[pk] = hh;
@@ -330,52 +390,37 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
       }
       __syncthreads();
       if (ntl == 0) continue;
-      // the first tile's slice of Gp, row counts and granule bases, requested ahead
-      int hcur = tl_h[0];
-      long long gb = (long long)hcur * tile4 + t;
+      // tile 0's list, then the requests for its slice and tile 1's counts
+      request_meta(tl_h[0]);
+      build(list_set(0), tl_c0[0], tl_n[0]);
+      long long gb = (long long)tl_h[0] * tile4 + t;
       float4 g0 = g4[gb], g1 = g4[gb + BLK_THREADS], g2 = g4[gb + 2 * BLK_THREADS],
              g3 = g4[gb + 3 * BLK_THREADS];
-      int cn = t < rbe ? (int)a.cnt[(long long)hcur * a.m_pad + p0 + t] : 0;
-      long long sbw = 0, nzw = 0, nz0 = 0;
-      if (wrows) {
-        const long long o = (long long)hcur * (a.nG + 1);
-        sbw = a.seg_base[o + j0 + w];
-        nzw = a.nz_base[o + j0 + w];
-        nz0 = a.nz_base[o + j0];
-      }
-      // entry batches: bq holds the next batch this lane works on; pf says
-      // that it was requested during the previous tile
+      request_meta(tl_h[min(1, ntl - 1)]);
+      __syncthreads();                     // tile 0's list is complete
+      // bq holds the first batch of this slot's run in the current tile
       Raw bq[U];
-      bool pf = false;
+      {
+        int e, e_end;
+        run_of(list_set(0), &e, &e_end);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          bq[u] = BlkEnt<VT>::fetch(a, e + 4 * u < e_end ? tl_c0[0] + e + 4 * u : a.zero_quad);
+      }
       for (int it = 0; it < ntl; it++) {
         const long long c0 = tl_c0[it];
-        const int tc = tl_n[it];
+        const bool has_next = it + 1 < ntl;
+        const BlkList Lc = list_set(it & 1);
+        const BlkList Ln = list_set((it + 1) & 1);
         __syncthreads();                   // the previous tile's readers are done
         float4* x4 = reinterpret_cast<float4*>(xs);
         x4[t] = g0;
         x4[t + BLK_THREADS] = g1;
         x4[t + 2 * BLK_THREADS] = g2;
         x4[t + 3 * BLK_THREADS] = g3;
-        // the compacted list of the block's rows with entries in this tile, their
-        // segment ends, and the row holding each slot's first entry
-        if (wrows) {
-          const int pc = (int)(sbw - c0) + blk_wave_scan(cn) - cn;
-          const int pn = (int)(nzw - nz0) + __popcll(__ballot(cn > 0) & lt_mask);
-          if (cn > 0) {
-            nz_row[pn] = t;
-            nz_end[pn] = pc + cn;
-            const unsigned nq = (unsigned)tc >> 2;
-            // first slot whose run starts at or after pc
-            unsigned k = (((unsigned)pc >> 2 << a.lgNS) + nq - 1) / nq;
-            for (; k < (unsigned)NS; k++) {
-              const int es = (int)(((nq * k) >> a.lgNS) << 2);
-              if (es >= pc + cn) break;
-              slot_k[k] = pn;
-            }
-          }
-        }
-        // the next tile's slice, counts and bases are requested before this
-        // tile's work (the last tile requests its own again)
+        // the next tile's list, then the requests for its slice and the counts
+        // of the one after
+        if (has_next) build(Ln, tl_c0[it + 1], tl_n[it + 1]);
         {
           const int hn = tl_h[min(it + 1, ntl - 1)];
           gb = (long long)hn * tile4 + t;
@@ -383,41 +428,34 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
           g1 = g4[gb + BLK_THREADS];
           g2 = g4[gb + 2 * BLK_THREADS];
           g3 = g4[gb + 3 * BLK_THREADS];
-          cn = t < rbe ? (int)a.cnt[(long long)hn * a.m_pad + p0 + t] : 0;
-          if (wrows) {
-            const long long o = (long long)hn * (a.nG + 1);
-            sbw = a.seg_base[o + j0 + w];
-            nzw = a.nz_base[o + j0 + w];
-            nz0 = a.nz_base[o + j0];
-          }
+          request_meta(tl_h[min(it + 2, ntl - 1)]);
         }
+        __syncthreads();
         int e, e_end;
-        blk_run(tc, ks, a.lgNS, &e, &e_end);
-        // the next tile's run start, for the batch requested ahead
+        run_of(Lc, &e, &e_end);
+        // the next tile's run, for the batch requested ahead
         long long c0n = 0;
         int en = 0, en_end = 0;
-        const bool has_next = it + 1 < ntl;
         if (has_next) {
           c0n = tl_c0[it + 1];
-          blk_run(tl_n[it + 1], ks, a.lgNS, &en, &en_end);
+          run_of(Ln, &en, &en_end);
         }
-        if (e < e_end && !pf) {
-#pragma unroll
-          for (int u = 0; u < U; u++)
-            bq[u] = BlkEnt<VT>::fetch(a, e + 4 * u < e_end ? c0 + e + 4 * u : a.zero_quad);
-        }
-        pf = false;
         if (e >= e_end && has_next) {
           // nothing to do here: request the next tile's first batch now
 #pragma unroll
           for (int u = 0; u < U; u++)
             bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
-          pf = true;
         }
-        __syncthreads();
-        int kk = e < e_end ? slot_k[ks] : 0;
-        int row = nz_row[kk];
-        int rend = nz_end[kk];
+        // the current row, the next one read ahead, and the current row's sums
+        int kk = Lc.slot[ks];
+        int row = Lc.row[kk], rend = Lc.start[kk + 1];
+        int row_n = Lc.row[kk + 1], rend_n = Lc.start[kk + 2];
+        float4 old[P / 4];
+        if (e < e_end) {
+#pragma unroll
+          for (int i = 0; i < P / 4; i++)
+            old[i] = reinterpret_cast<const float4*>(acc + ((size_t)row << a.lgFp) + q * P)[i];
+        }
         float a4[P];
 #pragma unroll
         for (int j = 0; j < P; j++) a4[j] = 0.0f;
@@ -438,7 +476,6 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
 #pragma unroll
               for (int u = 0; u < U; u++)
                 bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
-              pf = true;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -464,18 +501,26 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
                 }
               if (e < e_end) {
                 e += 4;
-                if (e == rend || e == e_end) {
-                  // the row (or this slot's run) ends: add its partial sums
-                  float* dst = acc + ((size_t)row << a.lgFp) + q * P;
+                if (e == rend) {
+                  // the row is complete in this tile: one slot owns it, so its
+                  // sums are read when it starts and written when it ends
+                  float4* dst = reinterpret_cast<float4*>(acc + ((size_t)row << a.lgFp) + q * P);
 #pragma unroll
-                  for (int j = 0; j < P; j++) {
-                    atomicAdd(dst + j, a4[j]);
-                    a4[j] = 0.0f;
+                  for (int i = 0; i < P / 4; i++) {
+                    dst[i] = make_float4(old[i].x + a4[4 * i + 0], old[i].y + a4[4 * i + 1],
+                                         old[i].z + a4[4 * i + 2], old[i].w + a4[4 * i + 3]);
                   }
-                  if (e == rend && e < e_end) {
+#pragma unroll
+                  for (int j = 0; j < P; j++) a4[j] = 0.0f;
+                  if (e < e_end) {
                     kk++;
-                    row = nz_row[kk];
-                    rend = nz_end[kk];
+                    row = row_n;
+                    rend = rend_n;
+                    row_n = Lc.row[kk + 1];
+                    rend_n = Lc.start[kk + 2];
+#pragma unroll
+                    for (int i = 0; i < P / 4; i++)
+                      old[i] = reinterpret_cast<const float4*>(acc + ((size_t)row << a.lgFp) + q * P)[i];
                   }
                 }
               }
