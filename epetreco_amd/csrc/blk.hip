@@ -168,6 +168,9 @@ template <> struct BlkEnt<ValF16> {
     v[1] = __half2float(__ushort_as_half((uint16_t)(pk.y >> 16)));
     v[2] = __half2float(__ushort_as_half((uint16_t)(pk.z >> 16)));
     v[3] = __half2float(__ushort_as_half((uint16_t)(pk.w >> 16)));
+    // one conversion per value, then full-rate FP32 FMAs (instead of the four
+    // mixed-precision FMAs per value the compiler would otherwise fold them into)
+    asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]));
   }
 };
 typedef unsigned short blk_us4 __attribute__((ext_vector_type(4)));

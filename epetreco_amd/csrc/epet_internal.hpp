@@ -177,6 +177,7 @@ constexpr int XB_BLOCKS = 8;
 // LDS tile of the LDS-tiled row products: 32768 floats = 128 KB of the 160 KB
 // of LDS per workgroup on gfx950
 constexpr int LDS_TILE = 32768;
+constexpr int LDS_CHUNK = 8;        // tile-major segments are whole chunks of 8 entries
 
 
 

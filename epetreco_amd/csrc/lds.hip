@@ -35,9 +35,46 @@ void launch_row_plan(hipStream_t st, int m, int T, int TS, const int64_t* rowptr
 constexpr int LDS_BLOCK = 1024;
 constexpr int LDS_WAVES = LDS_BLOCK / WAVE;
 
-// R rows per wavefront (L = 64/R lanes per row), J entries per lane per pass;
-// BWD = 1 marks the back projection over a transposed copy (a separate
-// instantiation, so that profiles tell the two products apart).
+// A chunk of LDS_CHUNK entries of the tile-major copy: eight 16-bit columns
+// (one 16-byte load) and eight values (one or two 16-byte loads).
+template <typename VT> struct LdsChunk;
+template <> struct LdsChunk<ValF32> {
+  static __device__ __forceinline__ void load(const float* __restrict__ val, long long k,
+                                              float v[8]) {
+    const float4 a = *reinterpret_cast<const float4*>(val + k);
+    const float4 b = *reinterpret_cast<const float4*>(val + k + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  }
+};
+template <> struct LdsChunk<ValBF16> {
+  static __device__ __forceinline__ void load(const uint16_t* __restrict__ val, long long k,
+                                              float v[8]) {
+    const uint4 a = *reinterpret_cast<const uint4*>(val + k);
+    const unsigned w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[2 * i] = __uint_as_float(w[i] << 16);
+      v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+    }
+  }
+};
+template <> struct LdsChunk<ValF16> {
+  static __device__ __forceinline__ void load(const uint16_t* __restrict__ val, long long k,
+                                              float v[8]) {
+    const uint4 a = *reinterpret_cast<const uint4*>(val + k);
+    const unsigned w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[2 * i] = __half2float(__ushort_as_half((uint16_t)(w[i] & 0xFFFFu)));
+      v[2 * i + 1] = __half2float(__ushort_as_half((uint16_t)(w[i] >> 16)));
+    }
+  }
+};
+
+// R rows per wavefront (L = 64/R lanes per row), J chunks of LDS_CHUNK entries
+// per lane per pass; BWD = 1 marks the back projection over a transposed copy
+// (a separate instantiation, so that profiles tell the two products apart).
 template <typename VT, int R, int J, int BWD>
 __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, long long rows_per_wg,
                                                         const int64_t* __restrict__ off,
@@ -69,7 +106,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
   const unsigned last = (unsigned)(cnt - 1);
   const long long step = (long long)LDS_WAVES * R;
   const int64_t* offt = off + (long long)t * m;
-  // segment [s0, s1) of row rbx + gi in tile t; empty past the range
+  // segment [s0, s1) of row rbx + gi in tile t (whole chunks); empty past the range
   auto segment = [&](long long rbx, long long& s0, long long& s1) {
     s0 = 0;
     s1 = 0;
@@ -88,25 +125,33 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_rows_lds(int m, int n, int T, lon
     const long long s0 = n0, s1 = n1;
     segment(rb + step, n0, n1);   // the next segment's bounds, requested before this one's entries
     float acc = 0.0f;
-    for (long long base = s0; base < s1; base += (long long)L * J) {
-      uint16_t c[J];
-      float v[J];
+    for (long long base = s0 + (long long)l * LDS_CHUNK; base < s1;
+         base += (long long)L * LDS_CHUNK * J) {
+      // J chunks: columns and values requested together (ordinary cached
+      // loads), chunks past the segment skipped
+      uint4 c[J];
+      float v[J][LDS_CHUNK];
       bool in[J];
 #pragma unroll
       for (int j = 0; j < J; j++) {
-        // ordinary (cached) loads: a row group reads part of a cache line per
-        // load and the rest with the next one
-        const long long k = base + l + (long long)j * L;
+        const long long k = base + (long long)j * L * LDS_CHUNK;
         in[j] = k < s1;
-        const long long kc = in[j] ? k : s1 - 1;
-        c[j] = col[kc];
-        v[j] = VT::get_cached(val, kc);
+        const long long kc = in[j] ? k : base;
+        c[j] = *reinterpret_cast<const uint4*>(col + kc);
+        LdsChunk<VT>::load(val, kc, v[j]);
       }
 #pragma unroll
       for (int j = 0; j < J; j++) {
-        // the clamp keeps LDS reads in range whatever the data
-        const unsigned li = std::min((unsigned)c[j], last);
-        acc += in[j] ? v[j] * xs[li] : 0.0f;
+        const unsigned cw[4] = {c[j].x, c[j].y, c[j].z, c[j].w};
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          // the clamp keeps LDS reads in range whatever the data
+          const unsigned lo = std::min(cw[i] & 0xFFFFu, last), hi = std::min(cw[i] >> 16, last);
+          a += v[j][2 * i] * xs[lo];
+          a += v[j][2 * i + 1] * xs[hi];
+        }
+        acc += in[j] ? a : 0.0f;
       }
     }
 #pragma unroll
@@ -174,6 +219,9 @@ __global__ __launch_bounds__(256) void k_ysort_rows(int m, const int64_t* __rest
   }
 }
 
+// padded segment lengths: every (tile, row) segment is a whole number of
+// LDS_CHUNK-entry chunks, so a lane reads a chunk's columns and values with
+// 16-byte loads
 __global__ __launch_bounds__(256) void k_lds_counts(int m, int T, const uint32_t* __restrict__ toff,
                                                     int* __restrict__ cnt) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // i = t m + r
@@ -181,14 +229,15 @@ __global__ __launch_bounds__(256) void k_lds_counts(int m, int T, const uint32_t
   const int t = (int)(i / m);
   const long long r = i % m;
   const uint32_t* o = toff + r * (T + 1) + t;
-  cnt[i] = (int)(o[1] - o[0]);
+  cnt[i] = ((int)(o[1] - o[0]) + LDS_CHUNK - 1) & ~(LDS_CHUNK - 1);
 }
 
 __global__ void k_lds_set_last(int64_t* __restrict__ off, long long i, long long v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) off[i] = v;
 }
 
-// one wavefront per (tile, row) segment: copy values and tile-local columns
+// one wavefront per (tile, row) segment: copy values and tile-local columns,
+// then zero entries (column 0, value 0) up to the padded length
 template <typename T>
 __global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, int mask,
                                                   const int64_t* __restrict__ rowptr,
@@ -204,11 +253,14 @@ __global__ __launch_bounds__(256) void k_lds_fill(int m, int T_, int mask,
        i += nw) {
     const int t = (int)(i / m);
     const long long r = i % m;
-    const long long src = rowptr[r] + toff[r * (T_ + 1) + t];
-    const long long dst = off[i], len = off[i + 1] - dst;
-    for (long long k = lane; k < len; k += WAVE) {
-      col16[dst + k] = (uint16_t)(col[src + k] & mask);
-      val2[dst + k] = val[src + k];
+    const uint32_t* o = toff + r * (T_ + 1) + t;
+    const long long src = rowptr[r] + o[0];
+    const long long len = o[1] - o[0];
+    const long long dst = off[i], plen = off[i + 1] - dst;
+    for (long long k = lane; k < plen; k += WAVE) {
+      const bool in = k < len;
+      col16[dst + k] = in ? (uint16_t)(col[src + k] & mask) : (uint16_t)0;
+      val2[dst + k] = in ? val[src + k] : (T)0;
     }
   }
 }
@@ -282,13 +334,13 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
                                             A->lds_partial, A->lds_map);
 }
 
-// Rows per wavefront from the mean segment length.
+// Rows per wavefront from the mean segment length; two chunks per lane per pass.
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
-  if (seg >= 384.0) launch_rows_lds<VT, 2, 16, BWD>(ctx, A, x);
-  else if (seg >= 48.0) launch_rows_lds<VT, 4, 16, BWD>(ctx, A, x);
-  else launch_rows_lds<VT, 16, 16, BWD>(ctx, A, x);
+  if (seg >= 384.0) launch_rows_lds<VT, 2, 2, BWD>(ctx, A, x);
+  else if (seg >= 48.0) launch_rows_lds<VT, 4, 2, BWD>(ctx, A, x);
+  else launch_rows_lds<VT, 16, 2, BWD>(ctx, A, x);
 }
 
 template <int BWD>
@@ -366,18 +418,20 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, ColMap mp, int
   bool ok = T >= 1 &&
             hipMalloc((void**)&toff, sizeof(uint32_t) * (size_t)A->m * (T + 1)) == hipSuccess &&
             hipMalloc((void**)&cnt, sizeof(int) * (size_t)std::max(1ll, nseg)) == hipSuccess &&
-            hipMalloc((void**)&off, sizeof(int64_t) * (size_t)(nseg + 1)) == hipSuccess &&
-            hipMalloc(&val, vb * (size_t)std::max(1ll, A->nnz)) == hipSuccess &&
-            hipMalloc((void**)&col16, sizeof(uint16_t) * (size_t)std::max(1ll, A->nnz)) == hipSuccess;
+            hipMalloc((void**)&off, sizeof(int64_t) * (size_t)(nseg + 1)) == hipSuccess;
   int rc = EPET_OK;
   if (ok) {
     launch_row_plan(st, A->m, T, TV, A->rowptr, scol, toff);
     k_lds_counts<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(A->m, T, toff, cnt);
     long long tot = 0;
     rc = exclusive_scan_i32_to_i64(ctx, cnt, off, nseg, &tot);
-    if (rc == EPET_OK && tot != A->nnz)
+    if (rc == EPET_OK && (tot < A->nnz || tot > A->nnz + (long long)(LDS_CHUNK - 1) * nseg))
       rc = fail(EPET_ERR_STATE, __FILE__, __LINE__, "tile-major copy: entry count mismatch");
-    if (rc == EPET_OK) {
+    ok = rc == EPET_OK &&
+         hipMalloc(&val, vb * (size_t)std::max(1ll, tot)) == hipSuccess &&
+         hipMalloc((void**)&col16, sizeof(uint16_t) * (size_t)std::max(1ll, tot)) == hipSuccess;
+    if (rc == EPET_OK && !ok) (void)hipGetLastError();
+    if (ok) {
       k_lds_set_last<<<1, 64, 0, st>>>(off, nseg, tot);
       const unsigned g = (unsigned)std::max<long long>(
           1, std::min<long long>((nseg + 3) / 4, 16ll * ctx->num_cus));
