@@ -429,7 +429,7 @@ def main():
             cpu["generation"] = gen_cpu
 
     algo_frac = fwd_gbps / HBM_PEAK_GBPS
-    traffic = measured_traffic(args.workload, "k_rows_lds<ValF32,4,16,0>")
+    traffic = measured_traffic(args.workload, "k_rows_lds<ValF32,4,1,0>")
     out = {
         "metric": "MLEM frame-iter/s + SpMV HBM GB/s at 52^3/5.1M ch; batched BF16 vs plain FP32",
         "value": plain_fis,
@@ -455,7 +455,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": fwd_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": algo_frac, "traffic": traffic,
                      "frac_physical": (traffic / fwd_ms / 1e6 / HBM_PEAK_GBPS) if traffic else None,
-                     "kernel": "forward SpMV (k_rows_lds<ValF32,4,16,0> + k_lds_reduce, timed together)",
+                     "kernel": "forward SpMV (k_rows_lds<ValF32,4,1,0> + k_lds_reduce, timed together)",
                      "algorithmic_bytes": B_fwd, "avg_ms": fwd_ms},
         "cpu_baseline": cpu,
     }

@@ -322,9 +322,10 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
     attr = true;
   }
   // each workgroup owns its CU's LDS; several workgroups per CU in all, so
-  // that tiles and row ranges of unequal weight even out: 8 for the forward
-  // and 16 for the back projection were fastest at 52^3
-  const long long per_cu = BWD ? 16 : 8;
+  // that tiles and row ranges of unequal weight even out: 12 for the forward
+  // and 24 for the back projection at 52^3 (4 to 32 per CU were within a few
+  // percent of each other)
+  const long long per_cu = BWD ? 24 : 12;
   const long long G = std::max<long long>(1, (per_cu * ctx->num_cus + A->lds_T - 1) / A->lds_T);
   const long long rows_per_wg = std::max<long long>(LDS_WAVES * R, (A->m + G - 1) / G);
   const long long ng = (A->m + rows_per_wg - 1) / rows_per_wg;
@@ -334,13 +335,14 @@ static void launch_rows_lds(epet_ctx* ctx, const epet_matrix* A, const float* x)
                                             A->lds_partial, A->lds_map);
 }
 
-// Rows per wavefront from the mean segment length; two chunks per lane per pass.
+// Rows per wavefront from the mean segment length; one chunk per lane per pass
+// (at 52^3 two chunks per pass took 8.1 ms per MLEM iteration, one 6.9-7.2 ms).
 template <typename VT, int BWD>
 static void rows_lds_dispatch(epet_ctx* ctx, const epet_matrix* A, const float* x) {
   const double seg = (double)A->nnz / std::max(1.0, (double)A->m * A->lds_T);
-  if (seg >= 384.0) launch_rows_lds<VT, 2, 2, BWD>(ctx, A, x);
-  else if (seg >= 48.0) launch_rows_lds<VT, 4, 2, BWD>(ctx, A, x);
-  else launch_rows_lds<VT, 16, 2, BWD>(ctx, A, x);
+  if (seg >= 384.0) launch_rows_lds<VT, 2, 1, BWD>(ctx, A, x);
+  else if (seg >= 48.0) launch_rows_lds<VT, 4, 1, BWD>(ctx, A, x);
+  else launch_rows_lds<VT, 16, 1, BWD>(ctx, A, x);
 }
 
 template <int BWD>
