@@ -168,6 +168,167 @@ def measured_traffic(workload, kernel):
     return k["fetch_bytes"] * t["copy_scale_fetch"] + k["write_bytes"] * t["copy_scale_write"]
 
 
+def series_104(frames=8, niter=30, chunk=400_000):
+    """Dynamic series on the 104^3 grid of 0.5 mm voxels, all 5,140,980 channels,
+    32 rays per channel: `frames` frames of the seeded phantom with totals
+    log-uniform in [1e6, 1e7], `niter` MLEM iterations per frame, batched with
+    FP16 and with BF16 values (per-row FP32 scale), against FP32 storage (one
+    single-frame MLEM per frame) and against the FP64 oracle on the host (all
+    frames, run concurrently on the host cores).  The operator is built in
+    channel chunks for the data and the sensitivity, then over the measured
+    channels once per run, so that each storage format has the device to
+    itself; the oracle works on a host copy.  Prints one JSON line."""
+    import threading
+
+    import numpy as np
+    import torch
+    from epetreco_amd import geometry as G
+    from epetreco_amd import phantom as PH
+    from epetreco_amd.api import Context, OP_N, VAL_BF16, VAL_F16
+    from oracle import oracle as O
+
+    setup, grid, nrays = G.workload("large")
+    ctx = Context(0)
+    ctx.set_geometry(setup, grid, nrays=nrays, seed=0)
+    V, N, F = grid.nvoxels, setup.nchannels, frames
+    host = host_info()
+    imgs, totals = PH.dynamic_series(grid, "large", F, seed=0, counts=(1e6, 1e7))
+    Xt = torch.as_tensor(imgs.T.astype(np.float32), device="cuda").contiguous()
+    # data and sensitivity over all channels, chunk by chunk
+    say(f"104^3 series: {N} channels x {V} voxels, {nrays} rays, {F} frames")
+    t0 = time.time()
+    ybar = np.zeros((N, F))
+    sens = torch.zeros(V, dtype=torch.float64, device="cuda")
+    nnz_all = 0
+    for c0 in range(0, N, chunk):
+        ids = np.arange(c0, min(N, c0 + chunk), dtype=np.int32)
+        Ac = ctx.sm_build(ids)
+        nnz_all += Ac.nnz
+        ybar[c0:c0 + len(ids)] = Ac.spmm(OP_N, Xt).double().cpu().numpy()
+        sens += Ac.colsum_f64()
+        Ac.close()
+        del Ac
+        say(f"    channels {c0}..{min(N, c0 + chunk)}: {nnz_all} nonzeros so far")
+    torch.cuda.synchronize()
+    gen_all_s = time.time() - t0
+    say(f"  all channels: {nnz_all} nonzeros, built in chunks in {gen_all_s:.1f} s")
+    ybar *= totals / ybar.sum(0)
+    counts = np.random.default_rng(5).poisson(ybar).astype(np.float32)
+    rows = np.nonzero(counts.sum(1) > 0)[0].astype(np.int32)
+    Y = np.ascontiguousarray(counts[rows])
+    del ybar, counts
+    s = sens.float()
+    t0 = time.time()
+    A = ctx.sm_build(rows)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    m, nnz = A.m, A.nnz
+    say(f"  measured channels {m}, nonzeros {nnz}, built in {gen_s:.1f} s")
+    # host copy of the FP32 operator for the oracle
+    rp_t, col_t, val_t, _ = A.to_torch()
+    rp_h = rp_t.cpu()
+    del rp_t
+    col_h = col_t.cpu()
+    del col_t
+    val_h = val_t.cpu()
+    del val_t
+    torch.cuda.empty_cache()
+    s_h = s.cpu().numpy()
+    x0 = np.full((V, F), 1.0 / V)
+    oracle_out = {}
+
+    def run_oracle():
+        # one iteration per call (an iteration depends on the image only)
+        t = time.time()
+        Xo = x0
+        for it in range(niter):
+            Xo = O.mlem_f64_frames_par(m, V, val_h.numpy(), rp_h.numpy(), col_h.numpy(), Y, s_h,
+                                       Xo, 1)
+            say(f"    FP64 oracle iteration {it + 1}/{niter}: {time.time() - t:.0f} s")
+        oracle_out["X"] = Xo
+        oracle_out["s"] = time.time() - t
+
+    th = threading.Thread(target=run_oracle)
+    th.start()
+    Yd = torch.as_tensor(Y, device="cuda")
+    # FP32 storage: single-frame MLEM per frame over A and its transposed copy
+    # (no LDS copies, so that both fit next to each other)
+    os.environ["EPET_LDS"] = "0"
+    try:
+        ctx32 = Context(0)
+    finally:
+        del os.environ["EPET_LDS"]
+    A.attach_transpose()
+    X32 = np.zeros((V, F), np.float32)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for f in range(F):
+        x = torch.full((V,), 1.0 / V, device="cuda")
+        ctx32.mlem(A, Yd[:, f].contiguous(), s, x, niter, use_graph=True)
+        X32[:, f] = x.cpu().numpy()
+    torch.cuda.synchronize()
+    fp32_ms = 1e3 * (time.time() - t0) / (F * niter)
+    say(f"  FP32 storage, single-frame: {fp32_ms:.1f} ms per frame-iteration")
+    A.close()
+    del A
+    torch.cuda.empty_cache()
+    out = {"series_104": True, "workload": f"{N} channels, {grid.nx}^3 voxels of {grid.dx * 1e3:g} mm, "
+                                             f"{nrays} rays per channel",
+           "channels": N, "voxels": V, "measured_channels": m, "nnz": nnz, "nnz_all_channels": nnz_all,
+           "frames": F, "iterations": niter, "frame_counts": [float(t) for t in totals],
+           "generation_all_channels_s": gen_all_s, "generation_measured_s": gen_s,
+           "fp32_single_frame_ms_per_frame_iteration": fp32_ms}
+    results = {}
+    for key, vt in (("fp16_row_scale", VAL_F16), ("bf16_row_scale", VAL_BF16)):
+        # rebuilt (deterministic generation) rather than uploaded: a generated
+        # matrix carries its channel ids and grid, which order the blocked copies
+        A32 = ctx.sm_build(rows)
+        M = A32.convert(vt)
+        A32.close()
+        del A32
+        M.attach_transpose()
+        X = torch.full((V, F), 1.0 / V, device="cuda")
+        t0 = time.time()
+        ctx.mlem(M, Yd, s, X, 1, use_graph=True)          # blocked copies and tuning
+        torch.cuda.synchronize()
+        prep_s = time.time() - t0
+        X.fill_(1.0 / V)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        ctx.mlem(M, Yd, s, X, niter, use_graph=True)
+        torch.cuda.synchronize()
+        t_b = time.time() - t0
+        Xh = X.cpu().numpy()
+        e32 = [float(np.linalg.norm(Xh[:, f] - X32[:, f]) / np.linalg.norm(X32[:, f])) for f in range(F)]
+        results[key] = {"ms_per_iteration": 1e3 * t_b / niter, "frame_iter_per_s": niter * F / t_b,
+                        "first_call_s": prep_s, "kernels": ctx.batched_config(M, F),
+                        "frame_rel_l2_vs_fp32_storage": e32, "X": Xh}
+        say(f"  {key}: {1e3 * t_b / niter:.1f} ms per {F}-frame iteration "
+            f"(first call {prep_s:.1f} s); rel-L2 vs FP32 storage " + " ".join(f"{e:.2e}" for e in e32))
+        M.close()
+        del M, X
+        torch.cuda.empty_cache()
+    th.join()
+    X64 = oracle_out["X"]
+    say(f"  FP64 oracle ({host['threads']} threads, concurrent): {oracle_out['s']:.0f} s")
+
+    def errs(Xa):
+        return [float(np.linalg.norm(Xa[:, f].astype(np.float64) - X64[:, f]) / np.linalg.norm(X64[:, f]))
+                for f in range(F)]
+
+    out["fp32_storage"] = {"frame_rel_l2_vs_fp64": errs(X32)}
+    say("  FP32 storage vs FP64: " + " ".join(f"{e:.2e}" for e in out["fp32_storage"]["frame_rel_l2_vs_fp64"]))
+    for key, r in results.items():
+        Xh = r.pop("X")
+        r["frame_rel_l2_vs_fp64"] = errs(Xh)
+        r["meets_1e-3_vs_fp64"] = bool(max(r["frame_rel_l2_vs_fp64"]) <= 1e-3)
+        say(f"  {key} vs FP64: " + " ".join(f"{e:.2e}" for e in r["frame_rel_l2_vs_fp64"]))
+        out[key] = r
+    out["oracle"] = {"seconds": oracle_out["s"], "threads": host["threads"], "host": host,
+                     "reference": "FP64 oracle MLEM on the FP32 matrix, all frames"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +348,9 @@ def main():
                     help="after building the problem, convert to this storage (bf16, f16, f32), "
                          "tune the batched products for --frames frames and run 3 batched MLEM "
                          "iterations (for kernel traces and counter runs)")
+    ap.add_argument("--series-104", action="store_true",
+                    help="the 104^3 dynamic series (all channels, 32 rays, 8 frames, 30 iterations, "
+                         "FP16 and BF16 storage against FP32 storage and the FP64 oracle); one JSON line")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="after building the problem, launch only the forward SpMV, the back "
                          "projection and a 1 GiB device copy (3 times each) for counter runs")
@@ -196,6 +360,10 @@ def main():
                                    "the row-sharded multi-GPU design is described, not run",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.series_104:
+        series_104()
+        return
 
     import numpy as np
     import torch
