@@ -203,20 +203,21 @@ template <> struct BlkEnt<ValF32> {
 //                    compacted, in position order,
 //             start  rb + 4 ints: their segment starts (start[rows] = the
 //                    tile's entry count),
-//             slot   BLK_MAX_SLOTS + 4 ints: each slot's first row (slot[NS]
-//                    = rows);
+//             slot   NS + 4 ints: each slot's first row (slot[NS] = rows),
+//                    NS = the workgroup's slots;
 //   tl_c0     BLK_TL int64, tl_h and tl_n BLK_TL ints: the non-empty tiles of
 //             a chunk of BLK_TL tiles, the block's first entry and its entry
 //             count in each;
 //   wsum      2 x 16 + 2 ints; ll 64 doubles.
-constexpr int BLK_MAX_SLOTS = BLK_WAVES * WAVE;   // one lane per slot at most
 constexpr int BLK_TL = 512;                        // tiles per chunk of the tile list
-__host__ __device__ inline int blk_list_ints(int rb) { return 2 * (rb + 4) + BLK_MAX_SLOTS + 4; }
+__host__ __device__ inline int blk_list_ints(int rb, int ns) { return 2 * (rb + 4) + ns + 4; }
+// slots of a workgroup: BLK_WAVES wavefronts of 64 / (Fp / P) slots
+__host__ __device__ inline int blk_slots(int Fp, int P) { return BLK_WAVES * (WAVE / (Fp / P)); }
 constexpr int BLK_LDS_TAIL = sizeof(int64_t) * BLK_TL + 2 * sizeof(int) * BLK_TL +
                              sizeof(int) * (2 * BLK_WAVES + 2) + sizeof(double) * 64;
-__host__ __device__ inline size_t blk_lds_bytes(int C, int Fp, int rb) {
+__host__ __device__ inline size_t blk_lds_bytes(int C, int Fp, int rb, int ns) {
   return sizeof(float) * ((size_t)C * Fp + (size_t)rb * Fp) +
-         sizeof(int) * 2 * (size_t)blk_list_ints(rb) + BLK_LDS_TAIL;
+         sizeof(int) * 2 * (size_t)blk_list_ints(rb, ns) + BLK_LDS_TAIL;
 }
 
 // Workgroup-wide exclusive scan of (a, b) pairs in thread order: returns this
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   float* xs = reinterpret_cast<float*>(blk_lds);
   float* acc = xs + ((size_t)a.C << a.lgFp);
   int* lists = reinterpret_cast<int*>(acc + ((size_t)a.rb << a.lgFp));
-  const int li = blk_list_ints(a.rb);
+  const int li = blk_list_ints(a.rb, 1 << a.lgNS);
   // list set s (0 or 1)
   auto list_set = [&](int s) {
     BlkList L;
@@ -1008,7 +1009,7 @@ static BlkKern blk_kernel_vt(int vt, int P) {
 // divide a wavefront and the workgroup's LDS holds the tile and the row sums.
 static bool blk_shape_valid(const BlkCopy* B, int P, int rb) {
   if (P > B->Fp || B->Fp / P > WAVE || rb % BLK_G != 0 || rb > BLK_MAX_RB) return false;
-  return blk_lds_bytes(B->C, B->Fp, rb) <= (size_t)BLK_LDS_LIMIT;
+  return blk_lds_bytes(B->C, B->Fp, rb, blk_slots(B->Fp, P)) <= (size_t)BLK_LDS_LIMIT;
 }
 
 // work-item splits of a direction: eight work items per CU at least, so that
@@ -1087,7 +1088,7 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits
   }
   const long long nwork = (long long)a.nblocks * a.splits;
   const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(nwork, ctx->num_cus));
-  kern<<<grid, BLK_THREADS, blk_lds_bytes(B->C, B->Fp, rb), ctx->stream>>>(a);
+  kern<<<grid, BLK_THREADS, blk_lds_bytes(B->C, B->Fp, rb, 1 << a.lgNS), ctx->stream>>>(a);
   if (splits > 1) {
     a.mode = mode;
     k_blk_reduce<<<grid_for((long long)B->m_pad * F, 256, 16ll * ctx->num_cus), 256, 0,
@@ -1201,14 +1202,14 @@ int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
   // (53 and 86 ms against 35 ms per 32-frame forward product at 52^3)
   EPET_TRY(blk_gather_x(ctx, dirs[0], a.F, a.x));
   static const int Ps[2] = {4, 8};
-  static const int RBs[3] = {128, 256, 512};
+  static const int RBs[4] = {128, 256, 512, 1024};
   int rc = EPET_OK;
   for (int d = 0; d < 2 && rc == EPET_OK; d++) {
     BlkCopy* B = dirs[d];
     float best = 1e30f;
     int bP = 0, bR = 0, bS = 1;
     for (int pi = 0; pi < 2 && rc == EPET_OK; pi++)
-      for (int ri = 0; ri < 3 && rc == EPET_OK; ri++) {
+      for (int ri = 0; ri < 4 && rc == EPET_OK; ri++) {
         const int P = Ps[pi], rb = RBs[ri];
         if (!blk_shape_valid(B, P, rb)) continue;
         const int splits = blk_splits(B, rb, ctx->num_cus);

@@ -266,7 +266,7 @@ int epet_mlem_noalloc(epet_ctx* ctx, const epet_matrix* A, const float* y_dev,
 
 /* Autotuner of the multi-frame products epet_mlem runs for F frames on A and
  * its attached transpose.  Blocked products (the default): every shape of
- * (rows per block rb in {128,256,512}) x (frames per lane P in {4,8}) that
+ * (rows per block rb in {128,256,512,1024}) x (frames per lane P in {4,8}) that
  * fits the workgroup's LDS is timed per direction and the fastest kept;
  * cfg = {rb_fwd, P_fwd, rb_bwd, P_bwd}.  Gather kernels
  * (EPET_BATCHED=gather, or blocked copies that cannot be allocated): (rows per

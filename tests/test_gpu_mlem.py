@@ -178,8 +178,8 @@ def test_multiframe_autotuner(small_problem, monkeypatch):
     A = Matrix.from_csr(ctx, rp, col, val, prob.A.n).attach_transpose()
     (rf, Pf, rbk, Pb), (mf, mb) = ctx.mlem_tune(A, 16)
     _say("blocked: fwd rb,P =", rf, Pf, f"{mf:.3f} ms; bwd rb,P =", rbk, Pb, f"{mb:.3f} ms")
-    assert rf in (128, 256, 512) and Pf in (4, 8)
-    assert rbk in (128, 256, 512) and Pb in (4, 8)
+    assert rf in (128, 256, 512, 1024) and Pf in (4, 8)
+    assert rbk in (128, 256, 512, 1024) and Pb in (4, 8)
     cfg = ctx.batched_config(A, 16)
     assert cfg["forward"]["rows_per_block"] == rf and cfg["back_projection"]["frames_per_lane"] == Pb
     ctx_g = env_ctx(monkeypatch, EPET_BATCHED="gather")
