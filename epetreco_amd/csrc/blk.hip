@@ -59,8 +59,9 @@ constexpr int BLK_THREADS = 1024;
 constexpr int BLK_WAVES = BLK_THREADS / WAVE;
 constexpr int BLK_TILE_FLOATS = 16384;      // 64 KB of the gathered vector per tile
 constexpr int BLK_G = 64;                   // row granule of the segment bases
+constexpr int BLK_ENT_PAD = 64;             // zero entries after the stream (batches read past runs)
 constexpr int BLK_MAX_RB = BLK_THREADS;     // rows per block (one row per thread)
-constexpr int BLK_LDS_LIMIT = 160 * 1024 - 2048;   // dynamic LDS per workgroup (gfx950)
+constexpr int BLK_LDS_LIMIT = 160 * 1024 - 2048;   // LDS per workgroup (gfx950 has 160 KB)
 // entry quads per batch of a slot; a lane holds two batches (the one it works
 // on and the next, requested ahead), so fewer with 8 frames per lane, whose
 // gathered rows take twice the registers, and with FP32 values, whose
@@ -124,7 +125,6 @@ struct BlkArgs {
   const uint16_t* cnt;
   const void* ent;
   const float* eval;
-  long long zero_quad;         // index of four zero entries after the stream
   const float* Gp;             // gathered vector in column-position order, T C x Fp
   float* out;                  // forward: ratio in position order (m_pad x Fp);
                                // back projection: image (n x F, frame-minor)
@@ -194,8 +194,8 @@ template <> struct BlkEnt<ValF32> {
 };
 
 // ------------------------------------------------------------------ main kernel
-// Dynamic LDS of a workgroup (all of it dynamic, so the tile starts at LDS
-// address 0 and an entry's byte offset is its LDS address):
+// LDS of a workgroup (one array of BLK_LDS_LIMIT bytes; the tile starts at
+// LDS address 0, so an entry's byte offset is its LDS address):
 //   tile      C Fp floats (64 KB) of Gp
 //   acc       rb Fp floats: the block's row sums
 //   lists     two sets, the current tile's and the next one's, each of
@@ -294,7 +294,9 @@ __device__ __forceinline__ void blk_place(const BlkList& L, int t, int cn, int p
 
 template <typename VT, int P>
 __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
-  extern __shared__ float4 blk_lds[];
+  // one static array, so that its address (0) is known when compiling and the
+  // tile's byte offsets are LDS addresses as they are
+  __shared__ float4 blk_lds[BLK_LDS_LIMIT / 16];
   typedef typename BlkEnt<VT>::Raw Raw;
   constexpr int U = blk_u<VT, P>();             // entry quads per batch
   float* xs = reinterpret_cast<float*>(blk_lds);
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
         run_of(list_set(0), &e, &e_end);
 #pragma unroll
         for (int u = 0; u < U; u++)
-          bq[u] = BlkEnt<VT>::fetch(a, e + 4 * u < e_end ? tl_c0[0] + e + 4 * u : a.zero_quad);
+          bq[u] = BlkEnt<VT>::fetch(a, tl_c0[0] + e + 4 * u);
       }
       for (int it = 0; it < ntl; it++) {
         const long long c0 = tl_c0[it];
@@ -447,8 +449,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
         if (e >= e_end && has_next) {
           // nothing to do here: request the next tile's first batch now
 #pragma unroll
-          for (int u = 0; u < U; u++)
-            bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
+          for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0n + en + 4 * u);
         }
         // the current row, the next one read ahead, and the current row's sums
         int kk = Lc.slot[ks];
@@ -469,20 +470,19 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
             Raw cur[U];
 #pragma unroll
             for (int u = 0; u < U; u++) cur[u] = bq[u];
-            // the following batch: the run's next one, else the next tile's first
+            // the following batch: the run's next one, else the next tile's
+            // first (a batch may read up to 4 U - 1 entries past a run, which
+            // the padding after the entries keeps inside the array)
             if (e + 4 * U < e_end) {
 #pragma unroll
-              for (int u = 0; u < U; u++) {
-                const int eu = e + 4 * (U + u);
-                bq[u] = BlkEnt<VT>::fetch(a, eu < e_end ? c0 + eu : a.zero_quad);
-              }
+              for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0 + e + 4 * (U + u));
             } else if (has_next) {
 #pragma unroll
-              for (int u = 0; u < U; u++)
-                bq[u] = BlkEnt<VT>::fetch(a, en + 4 * u < en_end ? c0n + en + 4 * u : a.zero_quad);
+              for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0n + en + 4 * u);
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
+              if (e >= e_end) break;           // quads past the run are not worked
               unsigned c[4];
               float v[4];
               BlkEnt<VT>::decode(cur[u], cm, c, v);
@@ -503,29 +503,27 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
                   a4[4 * i + 2] += v[t4] * xv[t4][i].z;
                   a4[4 * i + 3] += v[t4] * xv[t4][i].w;
                 }
-              if (e < e_end) {
-                e += 4;
-                if (e == rend) {
-                  // the row is complete in this tile: one slot owns it, so its
-                  // sums are read when it starts and written when it ends
-                  float4* dst = reinterpret_cast<float4*>(acc + ((size_t)row << a.lgFp) + q * P);
+              e += 4;
+              if (e == rend) {
+                // the row is complete in this tile: one slot owns it, so its
+                // sums are read when it starts and written when it ends
+                float4* dst = reinterpret_cast<float4*>(acc + (row << a.lgFp) + q * P);
 #pragma unroll
-                  for (int i = 0; i < P / 4; i++) {
-                    dst[i] = make_float4(old[i].x + a4[4 * i + 0], old[i].y + a4[4 * i + 1],
-                                         old[i].z + a4[4 * i + 2], old[i].w + a4[4 * i + 3]);
-                  }
+                for (int i = 0; i < P / 4; i++) {
+                  dst[i] = make_float4(old[i].x + a4[4 * i + 0], old[i].y + a4[4 * i + 1],
+                                       old[i].z + a4[4 * i + 2], old[i].w + a4[4 * i + 3]);
+                }
 #pragma unroll
-                  for (int j = 0; j < P; j++) a4[j] = 0.0f;
-                  if (e < e_end) {
-                    kk++;
-                    row = row_n;
-                    rend = rend_n;
-                    row_n = Lc.row[kk + 1];
-                    rend_n = Lc.start[kk + 2];
+                for (int j = 0; j < P; j++) a4[j] = 0.0f;
+                if (e < e_end) {
+                  kk++;
+                  row = row_n;
+                  rend = rend_n;
+                  row_n = Lc.row[kk + 1];
+                  rend_n = Lc.start[kk + 2];
 #pragma unroll
-                    for (int i = 0; i < P / 4; i++)
-                      old[i] = reinterpret_cast<const float4*>(acc + ((size_t)row << a.lgFp) + q * P)[i];
-                  }
+                  for (int i = 0; i < P / 4; i++)
+                    old[i] = reinterpret_cast<const float4*>(acc + (row << a.lgFp) + q * P)[i];
                 }
               }
             }
@@ -729,9 +727,10 @@ __global__ __launch_bounds__(256) void k_blk_fill(int m_pad, int T, int lgC, int
   }
 }
 
-__global__ void k_blk_zero_quad(void* ent, float* eval, long long at, int eb) {
+// zero entries [at, at + BLK_ENT_PAD) after the stream
+__global__ void k_blk_zero_tail(void* ent, float* eval, long long at, int eb) {
   const int i = threadIdx.x;
-  if (i < 4) {
+  if (i < BLK_ENT_PAD) {
     if (eb == 4) static_cast<uint32_t*>(ent)[at + i] = 0u;
     else static_cast<uint16_t*>(ent)[at + i] = 0;
     if (eval) eval[at + i] = 0.0f;
@@ -957,17 +956,17 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
       delete B;
       return rc;
     }
-    // entries, then one zero quad that reads past a run are pointed at
+    // entries, then BLK_ENT_PAD zero entries that batches reading past a run reach
     const size_t eb = M->value_type == EPET_VAL_F32 ? 2 : 4;
-    if (hipMalloc(&B->ent, eb * (size_t)(tot + 4)) != hipSuccess ||
+    if (hipMalloc(&B->ent, eb * (size_t)(tot + BLK_ENT_PAD)) != hipSuccess ||
         (M->value_type == EPET_VAL_F32 &&
-         hipMalloc((void**)&B->eval, sizeof(float) * (size_t)(tot + 4)) != hipSuccess)) {
+         hipMalloc((void**)&B->eval, sizeof(float) * (size_t)(tot + BLK_ENT_PAD)) != hipSuccess)) {
       (void)hipGetLastError();
       cleanup();
       delete B;
       return EPET_OK;
     }
-    k_blk_zero_quad<<<1, 64, 0, st>>>(B->ent, B->eval, tot, (int)eb);
+    k_blk_zero_tail<<<1, BLK_ENT_PAD, 0, st>>>(B->ent, B->eval, tot, (int)eb);
     if (M->value_type == EPET_VAL_F32)
       k_blk_fill<float><<<g, 256, lds, st>>>(B->m_pad, B->T, B->lgC, B->lgFp, B->rowperm, M->rowptr, M->col,
                                              (const float*)M->val, cpos_d, off, B->ent, B->eval);
@@ -1068,7 +1067,6 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits
   a.cnt = B->cnt;
   a.ent = B->ent;
   a.eval = B->eval;
-  a.zero_quad = B->nent;
   a.Gp = B->gp;
   a.out = out;
   a.partial = B->partial;
@@ -1078,17 +1076,9 @@ static int blk_launch(epet_ctx* ctx, const BlkCopy* B, int P, int rb, int splits
   a.Y = Y;
   a.itp = itp;
   a.loglik = loglik;
-  // every kernel may take the largest dynamic LDS of a workgroup (set once)
-  static bool attr_set[3][4] = {};
-  bool& done = attr_set[B->vt][ilog2(P)];
-  if (!done) {
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              BLK_LDS_LIMIT);
-    done = true;
-  }
   const long long nwork = (long long)a.nblocks * a.splits;
   const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(nwork, ctx->num_cus));
-  kern<<<grid, BLK_THREADS, blk_lds_bytes(B->C, B->Fp, rb, 1 << a.lgNS), ctx->stream>>>(a);
+  kern<<<grid, BLK_THREADS, 0, ctx->stream>>>(a);
   if (splits > 1) {
     a.mode = mode;
     k_blk_reduce<<<grid_for((long long)B->m_pad * F, 256, 16ll * ctx->num_cus), 256, 0,
