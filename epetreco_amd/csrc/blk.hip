@@ -22,12 +22,14 @@
 //
 // A workgroup owns a row block and keeps its rows' F-frame sums in LDS while
 // it walks the column tiles: for every tile the block touches it stages the
-// tile's C columns x Fp frames of G in LDS, then splits the block's entries in
-// that tile evenly over its slots of L lanes (L = Fp / P, P frames per lane):
-// each slot walks a contiguous run of entry quads, accumulates in registers
-// and adds a row's partial sum into the LDS row sums (LDS float atomics) when
-// the row or the run ends.  Only a quarter of a block's rows meet a given tile,
-// so entries, not rows, are what is balanced.  Loads are requested a step
+// tile's C columns x Fp frames of G in LDS, then shares the block's entries in
+// that tile out over its slots of L lanes (L = Fp / P, P frames per lane):
+// slot k takes the rows that start in its equal share of the tile's entry
+// quads, walks their entries, accumulates in registers, and adds a row's
+// partial sums into the LDS row sums when the row ends (a plain read at the
+// row's start and write at its end: no other slot touches the row in this
+// tile).  Only a quarter of a block's rows meet a given tile, so entries, not
+// rows, are what is balanced.  Loads are requested a step
 // ahead: the next tile's slice of G, counts and bases while the current tile
 // is worked, and each slot's next batch of entry quads (within the run, or
 // the first of its run in the next tile) while the current batch is.  The
