@@ -272,11 +272,11 @@ def test_lds_tiled_products_and_mlem(gpu_ctx_factory, oracle, storage, monkeypat
     assert rel_l2(x1.cpu().numpy(), x64) < REL_L2
 
 
-@pytest.mark.parametrize("F", [5, 13, 16, 32])
+@pytest.mark.parametrize("F", [2, 5, 13, 16, 32, 64])
 def test_multiframe_blocked_path(gpu_ctx_factory, oracle, monkeypatch, F):
-    """Multi-frame MLEM through the blocked register-accumulated products
-    (padded frame counts 8, 16 and 32; several column tiles each way; the back
-    projection split over workgroups) on long rows over 70,001 columns, FP32
+    """Multi-frame MLEM through the blocked products (padded frame counts 8,
+    16, 32 and 64; several column tiles each way; the back projection split
+    over workgroups) on long rows over 70,001 columns, FP32
     values: images after 6 iterations vs the gather kernels (rel-L2 <= 1e-5)
     and vs the multi-frame FP64 oracle (<= 1e-4), and the per-frame
     log-likelihood of the last iteration vs the gather kernels."""
