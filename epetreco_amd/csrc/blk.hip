@@ -67,9 +67,10 @@ constexpr int BLK_LDS_LIMIT = 160 * 1024 - 2048;   // LDS per workgroup (gfx950 
 // entry quads per batch of a slot; a lane holds two batches (the one it works
 // on and the next, requested ahead), so fewer with 8 frames per lane, whose
 // gathered rows take twice the registers, and with FP32 values, whose
-// entries take 6 bytes
+// entries take 6 bytes (with 16-bit values and 4 frames per lane, 3 quads
+// ran the 32-frame forward product 2% faster than 4)
 template <typename VT, int P> constexpr int blk_u() {
-  return sizeof(typename VT::T) == 4 ? (P == 4 ? 2 : 1) : (P == 4 ? 4 : 2);
+  return sizeof(typename VT::T) == 4 ? (P == 4 ? 2 : 1) : (P == 4 ? 3 : 2);
 }
 
 // Blocked copy of one matrix for one padded frame count.
@@ -201,10 +202,10 @@ template <> struct BlkEnt<ValF32> {
 //   tile      C Fp floats (64 KB) of Gp
 //   acc       rb Fp floats: the block's row sums
 //   lists     two sets, the current tile's and the next one's, each of
-//             row    rb + 4 ints: the block's rows with entries in the tile,
-//                    compacted, in position order,
-//             start  rb + 4 ints: their segment starts (start[rows] = the
-//                    tile's entry count),
+//             rs     rb + 4 int pairs: the block's rows with entries in the
+//                    tile, compacted, in position order, as (LDS byte
+//                    address of the row's sums, end of its segment) after a
+//                    leading (0, 0), so that row k starts at rs[k].y,
 //             slot   NS + 4 ints: each slot's first row (slot[NS] = rows),
 //                    NS = the workgroup's slots;
 //   tl_c0     BLK_TL int64, tl_h and tl_n BLK_TL ints: the non-empty tiles of
@@ -264,22 +265,21 @@ __device__ __forceinline__ int blk_wave_scan(int x) {
 
 // One tile's row list (see the LDS layout above).
 struct BlkList {
-  int* row;
-  int* start;
+  int2* rs;   // rs[0] = (0, 0); rs[k + 1] = (LDS byte address of row k's sums, end of row k)
   int* slot;
 };
 
-// Places row t (cn entries from pc, compacted index pn) of a tile of tc
-// entries in list L, and marks the slots whose first row it is.  Slot k's
-// share of the tile is the entry quads [es(k), es(k+1)), es(k) = 4
-// floor(k (tc / 4) / NS); it takes the rows that start in its share, so every
-// row is one slot's and no two slots add into one row of the sums.
-__device__ __forceinline__ void blk_place(const BlkList& L, int t, int cn, int pc, int pn, int tc,
+// Places row t (cn entries from pc, compacted index pn, its sums at LDS byte
+// address ra) of a tile of tc entries in list L, and marks the slots whose
+// first row it is.  Slot k's share of the tile is the entry quads [es(k),
+// es(k+1)), es(k) = 4 floor(k (tc / 4) / NS); it takes the rows that start in
+// its share, so every row is one slot's and no two slots add into one row of
+// the sums.  Row k starts where row k - 1 ends (rs[k].y).
+__device__ __forceinline__ void blk_place(const BlkList& L, int ra, int cn, int pc, int pn, int tc,
                                           int lgNS) {
   if (cn <= 0) return;
   const int NS = 1 << lgNS;
-  L.row[pn] = t;
-  L.start[pn] = pc;
+  L.rs[pn + 1] = make_int2(ra, pc + cn);
   const unsigned nq = (unsigned)tc >> 2;
   // first slot whose share starts at or after pc
   unsigned k = (((unsigned)pc >> 2 << lgNS) + nq - 1) / nq;
@@ -288,10 +288,7 @@ __device__ __forceinline__ void blk_place(const BlkList& L, int t, int cn, int p
     if (es >= pc + cn) break;
     L.slot[k] = es == pc ? pn : pn + 1;
   }
-  if (pc + cn == tc) {
-    L.start[pn + 1] = tc;
-    L.slot[NS] = pn + 1;
-  }
+  if (pc + cn == tc) L.slot[NS] = pn + 1;
 }
 
 template <typename VT, int P>
@@ -308,9 +305,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   // list set s (0 or 1)
   auto list_set = [&](int s) {
     BlkList L;
-    L.row = lists + s * li;
-    L.start = L.row + a.rb + 4;
-    L.slot = L.start + a.rb + 4;
+    L.rs = reinterpret_cast<int2*>(lists + s * li);
+    L.slot = lists + s * li + 2 * (a.rb + 4);
     return L;
   };
   int64_t* tl_c0 = reinterpret_cast<int64_t*>(lists + 2 * li);
@@ -325,12 +321,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
   const int ks = w * a.S + slot;           // this lane's slot
   const unsigned cm = 0xFFFFu;
   const int rbfp = a.rb << a.lgFp;
+  const int acc_addr = (a.C << a.lgFp) * (int)sizeof(float);   // LDS byte address of acc
   const int tile4 = (a.C << a.lgFp) >> 2;  // float4 per tile (4096)
   const unsigned lane_off = (unsigned)(q * P * 4);
   const char* xb = reinterpret_cast<const char*>(xs);
   const float4* g4 = reinterpret_cast<const float4*>(a.Gp);
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   if (t < 64) s_ll[t] = 0.0;
+  if (t < 2) list_set(t).rs[0] = make_int2(0, 0);
   double ll = 0.0;                         // log-likelihood of frame t mod Fp
   int* s_item = wsum + 2 * BLK_WAVES;     // the work item this workgroup took
   const int nwork = a.nblocks * a.splits;
@@ -369,13 +367,13 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
       if (wrows) {
         const int pc = (int)(sbw - c0x) + blk_wave_scan(cn) - cn;
         const int pn = (int)(nzw - nz0) + __popcll(__ballot(cn > 0) & lt_mask);
-        blk_place(L, t, cn, pc, pn, tcx, a.lgNS);
+        blk_place(L, acc_addr + (t << (a.lgFp + 2)), cn, pc, pn, tcx, a.lgNS);
       }
     };
     // this slot's rows and entries in a tile whose list is L
     auto run_of = [&](const BlkList& L, int* e0, int* e1) {
-      *e0 = L.start[L.slot[ks]];
-      *e1 = L.start[L.slot[ks + 1]];
+      *e0 = L.rs[L.slot[ks]].y;
+      *e1 = L.rs[L.slot[ks + 1]].y;
     };
     for (int hb = hlo; hb < hhi; hb += BLK_TL) {
       // the block's non-empty tiles among the next BLK_TL, in order, with its
@@ -406,14 +404,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
              g3 = g4[gb + 3 * BLK_THREADS];
       request_meta(tl_h[min(1, ntl - 1)]);
       __syncthreads();                     // tile 0's list is complete
-      // bq holds the first batch of this slot's run in the current tile
-      Raw bq[U];
+      // bA holds the first batch of this slot's run in the current tile
+      Raw bA[U];
       {
         int e, e_end;
         run_of(list_set(0), &e, &e_end);
 #pragma unroll
         for (int u = 0; u < U; u++)
-          bq[u] = BlkEnt<VT>::fetch(a, tl_c0[0] + e + 4 * u);
+          bA[u] = BlkEnt<VT>::fetch(a, tl_c0[0] + e + 4 * u);
       }
       for (int it = 0; it < ntl; it++) {
         const long long c0 = tl_c0[it];
@@ -441,95 +439,102 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
         __syncthreads();
         int e, e_end;
         run_of(Lc, &e, &e_end);
-        // the next tile's run, for the batch requested ahead
-        long long c0n = 0;
-        int en = 0, en_end = 0;
+        // where this slot's batch requests go once its run here is done: the
+        // first batch of its run in the next tile (after the last tile, any
+        // in-bounds address: a batch may read up to 4 U - 1 entries past a
+        // run, which the padding after the entries keeps inside the array)
+        long long nxt = c0 + e_end;
         if (has_next) {
-          c0n = tl_c0[it + 1];
+          int en, en_end;
           run_of(Ln, &en, &en_end);
+          nxt = tl_c0[it + 1] + en;
         }
-        if (e >= e_end && has_next) {
+        if (e >= e_end) {
           // nothing to do here: request the next tile's first batch now
 #pragma unroll
-          for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0n + en + 4 * u);
+          for (int u = 0; u < U; u++) bA[u] = BlkEnt<VT>::fetch(a, nxt + 4 * u);
         }
-        // the current row, the next one read ahead, and the current row's sums
-        int kk = Lc.slot[ks];
-        int row = Lc.row[kk], rend = Lc.start[kk + 1];
-        int row_n = Lc.row[kk + 1], rend_n = Lc.start[kk + 2];
-        float4 old[P / 4];
-        if (e < e_end) {
-#pragma unroll
-          for (int i = 0; i < P / 4; i++)
-            old[i] = reinterpret_cast<const float4*>(acc + ((size_t)row << a.lgFp) + q * P)[i];
-        }
+        // the current row (LDS byte address of its sums, end), the next one
+        // read ahead, and the current row's sums, read from the LDS row sums
+        // when the row starts, accumulated in registers and written back when
+        // it ends (one slot owns a row within a tile)
+        const int k0 = Lc.slot[ks];
+        int2 cr = Lc.rs[k0 + 1];
+        int2 nx = Lc.rs[k0 + 2];
+        const int2* kp = Lc.rs + k0 + 3;
         float a4[P];
+        auto load_row = [&]() {
 #pragma unroll
-        for (int j = 0; j < P; j++) a4[j] = 0.0f;
-        for (;;) {
-          if (!__any(e < e_end)) break;
-          if (e < e_end) {
-            Raw cur[U];
+          for (int i = 0; i < P / 4; i++) {
+            const float4 o = *reinterpret_cast<const float4*>(xb + cr.x + lane_off + 16 * i);
+            a4[4 * i + 0] = o.x;
+            a4[4 * i + 1] = o.y;
+            a4[4 * i + 2] = o.z;
+            a4[4 * i + 3] = o.w;
+          }
+        };
+        if (e < e_end) load_row();
+        // works the quads of batch b that lie in the run
+        auto work = [&](const Raw (&b)[U]) {
 #pragma unroll
-            for (int u = 0; u < U; u++) cur[u] = bq[u];
-            // the following batch: the run's next one, else the next tile's
-            // first (a batch may read up to 4 U - 1 entries past a run, which
-            // the padding after the entries keeps inside the array)
-            if (e + 4 * U < e_end) {
+          for (int u = 0; u < U; u++) {
+            if (e >= e_end) break;             // quads past the run are not worked
+            unsigned c[4];
+            float v[4];
+            BlkEnt<VT>::decode(b[u], cm, c, v);
+            // the quad's four gathered rows requested together, then accumulated
+            float4 xv[4][P / 4];
 #pragma unroll
-              for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0 + e + 4 * (U + u));
-            } else if (has_next) {
+            for (int t4 = 0; t4 < 4; t4++) {
+              const float4* xp = reinterpret_cast<const float4*>(xb + (c[t4] | lane_off));
 #pragma unroll
-              for (int u = 0; u < U; u++) bq[u] = BlkEnt<VT>::fetch(a, c0n + en + 4 * u);
+              for (int i = 0; i < P / 4; i++) xv[t4][i] = xp[i];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-              if (e >= e_end) break;           // quads past the run are not worked
-              unsigned c[4];
-              float v[4];
-              BlkEnt<VT>::decode(cur[u], cm, c, v);
-              // the quad's four gathered rows requested together, then accumulated
-              float4 xv[4][P / 4];
+            for (int t4 = 0; t4 < 4; t4++)
 #pragma unroll
-              for (int t4 = 0; t4 < 4; t4++) {
-                const float4* xp = reinterpret_cast<const float4*>(xb + (c[t4] | lane_off));
-#pragma unroll
-                for (int i = 0; i < P / 4; i++) xv[t4][i] = xp[i];
+              for (int i = 0; i < P / 4; i++) {
+                a4[4 * i + 0] += v[t4] * xv[t4][i].x;
+                a4[4 * i + 1] += v[t4] * xv[t4][i].y;
+                a4[4 * i + 2] += v[t4] * xv[t4][i].z;
+                a4[4 * i + 3] += v[t4] * xv[t4][i].w;
               }
+            e += 4;
+            if (e == cr.y) {
+              // the row is complete in this tile
+              float4* dst = reinterpret_cast<float4*>(const_cast<char*>(xb) + cr.x + lane_off);
 #pragma unroll
-              for (int t4 = 0; t4 < 4; t4++)
-#pragma unroll
-                for (int i = 0; i < P / 4; i++) {
-                  a4[4 * i + 0] += v[t4] * xv[t4][i].x;
-                  a4[4 * i + 1] += v[t4] * xv[t4][i].y;
-                  a4[4 * i + 2] += v[t4] * xv[t4][i].z;
-                  a4[4 * i + 3] += v[t4] * xv[t4][i].w;
-                }
-              e += 4;
-              if (e == rend) {
-                // the row is complete in this tile: one slot owns it, so its
-                // sums are read when it starts and written when it ends
-                float4* dst = reinterpret_cast<float4*>(acc + (row << a.lgFp) + q * P);
-#pragma unroll
-                for (int i = 0; i < P / 4; i++) {
-                  dst[i] = make_float4(old[i].x + a4[4 * i + 0], old[i].y + a4[4 * i + 1],
-                                       old[i].z + a4[4 * i + 2], old[i].w + a4[4 * i + 3]);
-                }
-#pragma unroll
-                for (int j = 0; j < P; j++) a4[j] = 0.0f;
-                if (e < e_end) {
-                  kk++;
-                  row = row_n;
-                  rend = rend_n;
-                  row_n = Lc.row[kk + 1];
-                  rend_n = Lc.start[kk + 2];
-#pragma unroll
-                  for (int i = 0; i < P / 4; i++)
-                    old[i] = reinterpret_cast<const float4*>(acc + (row << a.lgFp) + q * P)[i];
-                }
+              for (int i = 0; i < P / 4; i++)
+                dst[i] = make_float4(a4[4 * i + 0], a4[4 * i + 1], a4[4 * i + 2], a4[4 * i + 3]);
+              if (e < e_end) {
+                cr = nx;
+                nx = *kp++;
+                load_row();
               }
             }
           }
+        };
+        // the batch after the current one (starting at e): the run's next, or
+        // the next tile's first once the run ends in the current one
+        auto next_at = [&]() { return e + 4 * U < e_end ? c0 + e + 4 * U : nxt; };
+        // two batch buffers in turn, so that the batch requested ahead lands
+        // in the buffer not being worked (no register copies per batch); the
+        // loop leaves only at the top, with the next tile's first batch in bA
+        Raw bB[U];
+        for (;;) {
+          if (!__any(e < e_end)) break;
+          {
+            const long long at = next_at();
+#pragma unroll
+            for (int u = 0; u < U; u++) bB[u] = BlkEnt<VT>::fetch(a, at + 4 * u);
+          }
+          if (e < e_end) work(bA);
+          {
+            const long long at = next_at();
+#pragma unroll
+            for (int u = 0; u < U; u++) bA[u] = BlkEnt<VT>::fetch(a, at + 4 * u);
+          }
+          if (e < e_end) work(bB);
         }
       }
     }
