@@ -271,10 +271,12 @@ struct BlkList {
 
 // Places row t (cn entries from pc, compacted index pn, its sums at LDS byte
 // address ra) of a tile of tc entries in list L, and marks the slots whose
-// first row it is.  Slot k's share of the tile is the entry quads [es(k),
-// es(k+1)), es(k) = 4 floor(k (tc / 4) / NS); it takes the rows that start in
-// its share, so every row is one slot's and no two slots add into one row of
-// the sums.  Row k starts where row k - 1 ends (rs[k].y).
+// first row it is.  Slot k's equal share of the tile starts at entry es(k) =
+// 4 floor(k (tc / 4) / NS); its run starts at the row boundary nearest to
+// es(k) (the start of the row holding es(k) when es(k) lies in the row's first
+// half, else its end), so that every row is one slot's, no two slots add into
+// one row of the sums, and a run differs from the share by at most half a
+// row at either end.  Row k starts where row k - 1 ends (rs[k].y).
 __device__ __forceinline__ void blk_place(const BlkList& L, int ra, int cn, int pc, int pn, int tc,
                                           int lgNS) {
   if (cn <= 0) return;
@@ -286,7 +288,7 @@ __device__ __forceinline__ void blk_place(const BlkList& L, int ra, int cn, int 
   for (; k < (unsigned)NS; k++) {
     const int es = (int)(((nq * k) >> lgNS) << 2);
     if (es >= pc + cn) break;
-    L.slot[k] = es == pc ? pn : pn + 1;
+    L.slot[k] = 2 * (es - pc) <= cn ? pn : pn + 1;
   }
   if (pc + cn == tc) L.slot[NS] = pn + 1;
 }
