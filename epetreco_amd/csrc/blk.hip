@@ -24,11 +24,11 @@
 // it walks the column tiles: for every tile the block touches it stages the
 // tile's C columns x Fp frames of G in LDS, then shares the block's entries in
 // that tile out over its slots of L lanes (L = Fp / P, P frames per lane):
-// slot k takes the rows that start in its equal share of the tile's entry
-// quads, walks their entries, accumulates in registers, and adds a row's
-// partial sums into the LDS row sums when the row ends (a plain read at the
-// row's start and write at its end: no other slot touches the row in this
-// tile).  Only a quarter of a block's rows meet a given tile, so entries, not
+// slot k takes the rows between the row boundaries nearest to the ends of
+// its equal share of the tile's entry quads, walks their entries, and keeps
+// the current row's sums in registers: read from the LDS row sums when the
+// row starts and written back when it ends (no other slot touches the row
+// in this tile).  Only a quarter of a block's rows meet a given tile, so entries, not
 // rows, are what is balanced.  Loads are requested a step
 // ahead: the next tile's slice of G, counts and bases while the current tile
 // is worked, and each slot's next batch of entry quads (within the run, or
