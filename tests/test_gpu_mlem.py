@@ -310,6 +310,47 @@ def test_multiframe_blocked_path(gpu_ctx_factory, oracle, monkeypatch, F):
     np.testing.assert_allclose(ll1[-1], ll2[-1], rtol=1e-7)
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_multiframe_blocked_uneven_rows(gpu_ctx_factory, oracle, monkeypatch, storage):
+    """Blocked products on rows of very uneven length: most rows hold a few
+    entries, every 97th row tens of thousands, so that single rows span many
+    slots' shares of a (row block, tile) step, slot runs start at the nearest
+    row boundary on either side, and some slots have nothing to do.  16
+    frames, 5 MLEM iterations: images vs the gather kernels (rel-L2 <= 1e-5)
+    and vs the multi-frame FP64 oracle on the stored values (<= 1e-4)."""
+    from epetreco_amd.api import Matrix, VAL_F16
+    ctx = gpu_ctx_factory()
+    rng = np.random.default_rng(97)
+    m, n, F = 4000, 40000, 16
+    lens = rng.integers(1, 12, m)
+    lens[::97] = rng.integers(8000, 30000, len(lens[::97]))
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    val = (0.05 + rng.random(rp[-1])).astype(np.float32)
+    A = Matrix.from_csr(ctx, torch.as_tensor(rp), torch.as_tensor(col), torch.as_tensor(val), n)
+    if storage == "f16":
+        A = A.convert(VAL_F16)
+    A.attach_transpose()
+    from helpers import decoded
+    rq, cq, vq = decoded(A)
+    Y = rng.poisson(5.0, (m, F)).astype(np.float32)
+    s = torch.as_tensor(np.maximum(np.bincount(cq, weights=vq, minlength=n), 1e-3).astype(np.float32),
+                        device="cuda")
+    Yd = torch.as_tensor(Y, device="cuda")
+    X1 = torch.full((n, F), 1.0 / n, device="cuda")
+    ctx.mlem(A, Yd, s, X1, 5, use_graph=True)
+    cfg = ctx.batched_config(A, F)
+    _say("  uneven rows", storage, cfg)
+    assert cfg["path"].startswith("blocked")
+    ctx_g = env_ctx(monkeypatch, EPET_BATCHED="gather")
+    X2 = torch.full((n, F), 1.0 / n, device="cuda")
+    ctx_g.mlem(A, Yd, s, X2, 5, use_graph=True)
+    X64 = oracle.mlem_f64_frames(m, n, vq, rq, cq, Y, s.cpu().numpy(), np.full((n, F), 1.0 / n), 5)
+    for f in range(F):
+        assert rel_l2(X1[:, f].cpu().numpy(), X2[:, f].cpu().numpy().astype(np.float64)) < 1e-5
+        assert rel_l2(X1[:, f].cpu().numpy(), X64[:, f]) < REL_L2
+
+
 def test_voxel_renumbered_tiles(gpu_ctx_factory, monkeypatch):
     """Single-frame tile-major copies in y-major voxel order (EPET_LDS_MAP=1,
     for matrices over the context's grid): the projector workload's forward
