@@ -23,25 +23,24 @@
 // A workgroup owns a row block and keeps its rows' F-frame sums in LDS while
 // it walks the column tiles: for every tile the block touches it stages the
 // tile's C columns x Fp frames of G in LDS, then shares the block's entries in
-// that tile out over its slots of L lanes (L = Fp / P, P frames per lane):
-// slot k takes the rows between the row boundaries nearest to the ends of
-// its equal share of the tile's entry quads, walks their entries, and keeps
-// the current row's sums in registers: read from the LDS row sums when the
-// row starts and written back when it ends (no other slot touches the row
-// in this tile).  Only a quarter of a block's rows meet a given tile, so entries, not
-// rows, are what is balanced.  Loads are requested a step
-// ahead: the next tile's slice of G, counts and bases while the current tile
-// is worked, and each slot's next batch of entry quads (within the run, or
-// the first of its run in the next tile) while the current batch is.  The
-// matrix streams once, no
-// partial sums per (row, tile) reach memory, and the complete row sums feed
-// the epilogue: the MLEM ratio and the FP64 log-likelihood for the forward
-// projection, the image for the back projection.  When a direction has few
-// row blocks (the back projection's voxel rows), each block's tile range is
-// cut into `splits` equal parts for as many workgroups, which write partial
-// sums that a reduction adds in split order.  The grid is persistent (one workgroup per CU
-// looping over the work items), and the (rows per block, frames per lane)
-// shape is autotuned per matrix and frame count.
+// that tile out over its slots of L lanes (L = Fp / P, P frames per lane).
+// Only a quarter of a block's rows meet a given tile, so entries, not rows,
+// are what is balanced: slot k takes the rows between the row boundaries
+// nearest to the ends of its equal share of the tile's entry quads, walks
+// their entries, and keeps the current row's sums in registers, read from the
+// LDS row sums when the row starts and written back when it ends (no other
+// slot touches the row in this tile).  Loads are requested a step ahead: the
+// next tile's slice of G, counts and bases while the current tile is worked,
+// and each slot's next batch of entry quads (within the run, or the first of
+// its run in the next tile) while the current batch is.  The matrix streams
+// once, no partial sums per (row, tile) reach memory, and the complete row
+// sums feed the epilogue: the MLEM ratio and the FP64 log-likelihood for the
+// forward projection, the image for the back projection.  When a direction
+// has few row blocks (the back projection's voxel rows), each block's tile
+// range is cut into `splits` equal parts for as many workgroups, which write
+// partial sums that a reduction adds in split order.  The grid is persistent
+// (one workgroup per CU looping over the work items), and the (rows per
+// block, frames per lane) shape is autotuned per matrix and frame count.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
