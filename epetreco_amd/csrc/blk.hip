@@ -898,15 +898,22 @@ static int blk_build(epet_ctx* ctx, const epet_matrix* M, int Fp, const std::vec
     delete B;
     return EPET_OK;
   }
-  EPET_HIP(hipMemcpyAsync(B->rowperm, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice, st));
-  EPET_HIP(hipMemcpyAsync(B->colinv, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, st));
-  EPET_HIP(hipMemcpyAsync(cpos_d, cpos.data(), sizeof(int) * cpos.size(), hipMemcpyHostToDevice, st));
-  k_fill_i32_v<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(cnt32, nseg, 0);
-  k_fill_i32_v<<<1, 64, 0, st>>>(B->work, 2, 0);
-  k_fill_i32_v<<<grid_for(ngp, 256, 1ll << 31), 256, 0, st>>>(reinterpret_cast<int*>(B->gp), ngp, 0);
-  const size_t lds = sizeof(int) * (256 / WAVE) * (size_t)B->T;
   int rc = EPET_OK;
-  if (lds > 160 * 1024) {
+  if (hipMemcpyAsync(B->rowperm, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(B->colinv, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(cpos_d, cpos.data(), sizeof(int) * cpos.size(), hipMemcpyHostToDevice, st) !=
+          hipSuccess) {
+    rc = fail(EPET_ERR_HIP, __FILE__, __LINE__, "blocked copy: orders");
+  } else {
+    k_fill_i32_v<<<grid_for(nseg, 256, 1ll << 31), 256, 0, st>>>(cnt32, nseg, 0);
+    k_fill_i32_v<<<1, 64, 0, st>>>(B->work, 2, 0);
+    k_fill_i32_v<<<grid_for(ngp, 256, 1ll << 31), 256, 0, st>>>(reinterpret_cast<int*>(B->gp), ngp,
+                                                                0);
+  }
+  const size_t lds = sizeof(int) * (256 / WAVE) * (size_t)B->T;
+  if (rc == EPET_OK && lds > 160 * 1024) {
     rc = fail(EPET_ERR_UNSUPPORTED, __FILE__, __LINE__, "blocked copy: too many column tiles");
   }
   const unsigned g = (unsigned)std::max<long long>(
@@ -1192,16 +1199,17 @@ int blk_tune(epet_ctx* ctx, epet_matrix* A, const MfArgs& a, int reps) {
   const int Fp = padded_frames(a.F);
   BlkCopy* dirs[2] = {blk_find(A, Fp), blk_find(A->tr, Fp)};
   EPET_CHECK(dirs[0] && dirs[1], EPET_ERR_STATE, "blocked copies missing");
-  hipEvent_t e0, e1;
-  EPET_HIP(hipEventCreate(&e0));
-  EPET_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(EPET_ERR_HIP, __FILE__, __LINE__, "tuning events");
+  }
   // frames per lane: 4 (a slot of Fp/4 lanes reads whole voxel rows, no bank
   // conflicts) or 8; 16 and 32 lost to bank conflicts at every block size
   // (53 and 86 ms against 35 ms per 32-frame forward product at 52^3)
-  EPET_TRY(blk_gather_x(ctx, dirs[0], a.F, a.x));
   static const int Ps[2] = {4, 8};
   static const int RBs[4] = {128, 256, 512, 1024};
-  int rc = EPET_OK;
+  int rc = blk_gather_x(ctx, dirs[0], a.F, a.x);
   for (int d = 0; d < 2 && rc == EPET_OK; d++) {
     BlkCopy* B = dirs[d];
     float best = 1e30f;
