@@ -147,8 +147,9 @@ static int scan_impl(epet_ctx* ctx, const Tin* in, Tout* out, long long n, Tout*
   }
   const long long ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   EPET_CHECK(ntiles < (1ll << 31), EPET_ERR_UNSUPPORTED, "scan too long");
+  StreamTemps tmp(ctx->stream);
   Tout* sums = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&sums, sizeof(Tout) * (ntiles + 1), ctx->stream));
+  EPET_HIP(tmp.alloc(&sums, sizeof(Tout) * (ntiles + 1)));
   k_tile_sums<Tin, Tout><<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, sums);
   EPET_HIP(hipGetLastError());
   // scan of the tile sums; the total lands in sums[ntiles]
@@ -157,15 +158,13 @@ static int scan_impl(epet_ctx* ctx, const Tin* in, Tout* out, long long n, Tout*
     EPET_HIP(hipGetLastError());
   } else {
     Tout* total2 = sums + ntiles;
-    int r = scan_impl<Tout, Tout>(ctx, sums, sums, ntiles, total2);
-    if (r != EPET_OK) { (void)hipFreeAsync(sums, ctx->stream); return r; }
+    EPET_TRY((scan_impl<Tout, Tout>(ctx, sums, sums, ntiles, total2)));
   }
   k_tile_scan<Tin, Tout><<<(unsigned)ntiles, SCAN_BLOCK, 0, ctx->stream>>>(in, n, sums, out);
   EPET_HIP(hipGetLastError());
   if (total_dev)
     EPET_HIP(hipMemcpyAsync(total_dev, sums + ntiles, sizeof(Tout), hipMemcpyDeviceToDevice,
                             ctx->stream));
-  EPET_HIP(hipFreeAsync(sums, ctx->stream));
   return EPET_OK;
 }
 

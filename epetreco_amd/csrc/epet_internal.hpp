@@ -29,6 +29,53 @@ int fail(int code, const char* file, int line, const std::string& msg);
 #define EPET_TRY(call) \
   do { int _r = (call); if (_r != EPET_OK) return _r; } while (0)
 
+// ---------------------------------------------------------------- temporaries
+// Stream-ordered temporary device buffers, freed on the stream when the guard
+// leaves scope, so that every return path (error paths included) releases
+// them after the work enqueued before it.
+class StreamTemps {
+ public:
+  explicit StreamTemps(hipStream_t st) : st_(st) {}
+  ~StreamTemps() {
+    for (int i = 0; i < n_; i++) (void)hipFreeAsync(p_[i], st_);
+  }
+  StreamTemps(const StreamTemps&) = delete;
+  StreamTemps& operator=(const StreamTemps&) = delete;
+  template <typename T>
+  hipError_t alloc(T** out, size_t bytes) {
+    *out = nullptr;
+    if (n_ == kMax) return hipErrorOutOfMemory;
+    const hipError_t e = hipMallocAsync(reinterpret_cast<void**>(out), bytes, st_);
+    if (e == hipSuccess) p_[n_++] = *out;
+    return e;
+  }
+
+ private:
+  static constexpr int kMax = 8;
+  hipStream_t st_;
+  void* p_[kMax] = {};
+  int n_ = 0;
+};
+
+// A matrix under construction, destroyed unless released to the caller.
+class MatrixGuard {
+ public:
+  explicit MatrixGuard(epet_matrix* A) : A_(A) {}
+  ~MatrixGuard() {
+    if (A_) (void)epet_matrix_destroy(A_);
+  }
+  MatrixGuard(const MatrixGuard&) = delete;
+  MatrixGuard& operator=(const MatrixGuard&) = delete;
+  epet_matrix* release() {
+    epet_matrix* A = A_;
+    A_ = nullptr;
+    return A;
+  }
+
+ private:
+  epet_matrix* A_;
+};
+
 // ---------------------------------------------------------------- geometry
 // Everything a generation kernel needs, passed by value in the kernel-argument
 // segment (read-only, scalar-cache resident: the AMD home of per-launch

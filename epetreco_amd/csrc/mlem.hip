@@ -463,14 +463,13 @@ int epet_scales(epet_ctx* ctx, float* x, float alpha, long long n) {
 
 int epet_sum(epet_ctx* ctx, const float* x, long long n, double* out) {
   EPET_CHECK(ctx && out && (n == 0 || x), EPET_ERR_ARG, "null argument");
+  StreamTemps tmp(ctx->stream);
   double* d = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&d, sizeof(double), ctx->stream));
+  EPET_HIP(tmp.alloc(&d, sizeof(double)));
   k_zero_f64<<<1, 64, 0, ctx->stream>>>(d, 1);
   if (n > 0) k_sum_f64<<<grid_for(n, 256, 2048), 256, 0, ctx->stream>>>(x, n, d);
   EPET_HIP(hipGetLastError());
-  int rc = read_pinned(ctx, out, d, sizeof(double));
-  (void)hipFreeAsync(d, ctx->stream);
-  return rc;
+  return read_pinned(ctx, out, d, sizeof(double));
 }
 
 int epet_mlem(epet_ctx* ctx, const epet_matrix* A, int F, const float* y, const float* s_in,

@@ -425,9 +425,16 @@ static int generate_rows(epet_ctx* ctx, const Geo& g, const int* list, int m,
     rows = std::max<long long>(1, std::min<long long>(rows, 1 << 17));
     rows = std::min<long long>(rows, m - r0);
     if (rows > maxrows) {
-      if (cnt) { (void)hipFreeAsync(cnt, ctx->stream); (void)hipFreeAsync(off, ctx->stream); }
-      EPET_HIP(hipMallocAsync((void**)&cnt, sizeof(int) * rows, ctx->stream));
-      EPET_HIP(hipMallocAsync((void**)&off, sizeof(int64_t) * (rows + 1), ctx->stream));
+      if (cnt) (void)hipFreeAsync(cnt, ctx->stream);
+      if (off) (void)hipFreeAsync(off, ctx->stream);
+      cnt = nullptr;
+      off = nullptr;
+      if (hipMallocAsync((void**)&cnt, sizeof(int) * rows, ctx->stream) != hipSuccess ||
+          hipMallocAsync((void**)&off, sizeof(int64_t) * (rows + 1), ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = fail(EPET_ERR_ALLOC, __FILE__, __LINE__, "row counts");
+        break;
+      }
       maxrows = (int)rows;
     }
     void* scr = nullptr;
@@ -616,10 +623,11 @@ int epet_getWorkqueue(epet_ctx* ctx, const int* list_dev, int list_size, int* li
              "Invalid voxel index encountered during function 'getWorkqueueEntry(...)': " +
                  std::to_string(vi));
   const int rows = list_size - li;
+  StreamTemps tmp(ctx->stream);
   int* counts = nullptr;
   int64_t* offs = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&counts, sizeof(int) * rows, ctx->stream));
-  EPET_HIP(hipMallocAsync((void**)&offs, sizeof(int64_t) * (rows + 1), ctx->stream));
+  EPET_HIP(tmp.alloc(&counts, sizeof(int) * rows));
+  EPET_HIP(tmp.alloc(&offs, sizeof(int64_t) * (rows + 1)));
   k_wq_count<<<rows, GEN_BLOCK, 0, ctx->stream>>>(g, list_dev, li, vi, counts);
   EPET_HIP(hipGetLastError());
   long long total = 0;
@@ -629,15 +637,11 @@ int epet_getWorkqueue(epet_ctx* ctx, const int* list_dev, int list_size, int* li
   if (!out_cnl_dev && !out_vxl_dev && out_capacity == 0) {
     // count-only query: the number of pairs a call with this cap would emit;
     // the cursor is left unchanged
-    (void)hipFreeAsync(counts, ctx->stream);
-    (void)hipFreeAsync(offs, ctx->stream);
     EPET_HIP(hipStreamSynchronize(ctx->stream));
     *n_found = want;
     return EPET_OK;
   }
   if (want > out_capacity || (want > 0 && (!out_cnl_dev || !out_vxl_dev))) {
-    (void)hipFreeAsync(counts, ctx->stream);
-    (void)hipFreeAsync(offs, ctx->stream);
     EPET_FAIL(EPET_ERR_CAPACITY, "workqueue needs " + std::to_string(want) +
                                      " entries, output holds " + std::to_string(out_capacity));
   }
@@ -665,8 +669,6 @@ int epet_getWorkqueue(epet_ctx* ctx, const int* list_dev, int list_size, int* li
     *vxlId = 0;
     EPET_HIP(hipStreamSynchronize(ctx->stream));
   }
-  (void)hipFreeAsync(counts, ctx->stream);
-  (void)hipFreeAsync(offs, ctx->stream);
   *n_found = want;
   return EPET_OK;
 }
@@ -765,11 +767,16 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
       EPET_FAIL(EPET_ERR_ALLOC, "system matrix allocation of " + std::to_string(cap2) +
                                     " entries failed");
     }
-    if (used > 0) {
-      EPET_HIP(hipMemcpyAsync(c2, col, sizeof(int) * used, hipMemcpyDeviceToDevice, ctx->stream));
-      EPET_HIP(hipMemcpyAsync(v2, val, sizeof(float) * used, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((used > 0 &&
+         (hipMemcpyAsync(c2, col, sizeof(int) * used, hipMemcpyDeviceToDevice, ctx->stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(v2, val, sizeof(float) * used, hipMemcpyDeviceToDevice, ctx->stream) !=
+              hipSuccess)) ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      (void)hipFree(c2);
+      (void)hipFree(v2);
+      EPET_FAIL(EPET_ERR_HIP, "system matrix: growing the arrays");
     }
-    EPET_HIP(hipStreamSynchronize(ctx->stream));
     if (col) (void)hipFree(col);
     if (val) (void)hipFree(val);
     col = c2; val = v2; capacity = cap2;
@@ -813,11 +820,6 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
     (void)hipFree(rowptr);
     EPET_FAIL(EPET_ERR_ALLOC, "row id allocation failed");
   }
-  if (m > 0) {
-    k_row_ids<<<grid_for(m, 256, 1ll << 31), 256, 0, ctx->stream>>>(list_dev, m, ids);
-    EPET_HIP(hipGetLastError());
-    EPET_HIP(hipStreamSynchronize(ctx->stream));
-  }
   epet_matrix* A = new epet_matrix();
   A->m = m;
   A->n = g.V;
@@ -836,7 +838,13 @@ int epet_sm_build(epet_ctx* ctx, const int* list_dev, int m, epet_matrix** out) 
   A->mg.ny = g.gn[1];
   A->mg.nz = g.gn[2];
   A->device = ctx->device;
-  *out = A;
+  MatrixGuard keep(A);
+  if (m > 0) {
+    k_row_ids<<<grid_for(m, 256, 1ll << 31), 256, 0, ctx->stream>>>(list_dev, m, ids);
+    EPET_HIP(hipGetLastError());
+    EPET_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *out = keep.release();
   return EPET_OK;
 }
 

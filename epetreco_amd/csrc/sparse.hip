@@ -409,12 +409,13 @@ int epet_cooSort(epet_ctx* ctx, float* val, int* row, int* col, long long N) {
   int64_t* offs = nullptr;
   unsigned long long* andor = nullptr;
   hipStream_t st = ctx->stream;
-  EPET_HIP(hipMallocAsync((void**)&k0, sizeof(uint64_t) * N, st));
-  EPET_HIP(hipMallocAsync((void**)&k1, sizeof(uint64_t) * N, st));
-  EPET_HIP(hipMallocAsync((void**)&v1, sizeof(float) * N, st));
-  EPET_HIP(hipMallocAsync((void**)&hist, sizeof(int) * ntiles * RS_BINS, st));
-  EPET_HIP(hipMallocAsync((void**)&offs, sizeof(int64_t) * (ntiles * RS_BINS + 1), st));
-  EPET_HIP(hipMallocAsync((void**)&andor, sizeof(unsigned long long) * 2, st));
+  StreamTemps tmp(st);
+  EPET_HIP(tmp.alloc(&k0, sizeof(uint64_t) * N));
+  EPET_HIP(tmp.alloc(&k1, sizeof(uint64_t) * N));
+  EPET_HIP(tmp.alloc(&v1, sizeof(float) * N));
+  EPET_HIP(tmp.alloc(&hist, sizeof(int) * ntiles * RS_BINS));
+  EPET_HIP(tmp.alloc(&offs, sizeof(int64_t) * (ntiles * RS_BINS + 1)));
+  EPET_HIP(tmp.alloc(&andor, sizeof(unsigned long long) * 2));
   const unsigned g = grid_for(N, 256, 1ll << 31);
   k_pack_keys<<<g, 256, 0, st>>>(row, col, N, k0);
   k_init_andor<<<1, 64, 0, st>>>(andor);
@@ -442,12 +443,6 @@ int epet_cooSort(epet_ctx* ctx, float* val, int* row, int* col, long long N) {
   if (vin != val) EPET_HIP(hipMemcpyAsync(val, vin, sizeof(float) * N, hipMemcpyDeviceToDevice, st));
   k_unpack_keys<<<g, 256, 0, st>>>(kin, N, row, col);
   EPET_HIP(hipGetLastError());
-  (void)hipFreeAsync(k0, st);
-  (void)hipFreeAsync(k1, st);
-  (void)hipFreeAsync(v1, st);
-  (void)hipFreeAsync(hist, st);
-  (void)hipFreeAsync(offs, st);
-  (void)hipFreeAsync(andor, st);
   return rc;
 }
 
@@ -455,8 +450,9 @@ int epet_convertCoo2Csr(epet_ctx* ctx, int* csrRowPtr, const int* cooRowId, long
   EPET_CHECK(ctx && csrRowPtr && (nnz == 0 || cooRowId), EPET_ERR_ARG, "null argument");
   EPET_CHECK(m >= 0 && nnz >= 0 && nnz < (1ll << 31), EPET_ERR_ARG, "bad sizes");
   hipStream_t st = ctx->stream;
+  StreamTemps tmp(st);
   int* bad = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&bad, sizeof(int), st));
+  EPET_HIP(tmp.alloc(&bad, sizeof(int)));
   k_fill_i32<<<1, 64, 0, st>>>(bad, 1, 0);
   k_fill_i32<<<grid_for(m + 1, 256, 1ll << 31), 256, 0, st>>>(csrRowPtr, m + 1, 0);
   if (nnz > 0)
@@ -465,7 +461,6 @@ int epet_convertCoo2Csr(epet_ctx* ctx, int* csrRowPtr, const int* cooRowId, long
   EPET_TRY(exclusive_scan_i32(ctx, csrRowPtr, csrRowPtr, (long long)m + 1, nullptr));
   int hbad = 0;
   EPET_TRY(read_pinned(ctx, &hbad, bad, sizeof(int)));
-  (void)hipFreeAsync(bad, st);
   EPET_CHECK(!hbad, EPET_ERR_ARG, "COO row id outside [0, m)");
   return EPET_OK;
 }
@@ -520,19 +515,19 @@ int epet_matrix_create(epet_ctx* ctx, int m, int n, long long nnz, const int64_t
   hipStream_t st = ctx->stream;
   int hflags = 0;
   if (m > 0) {
+    StreamTemps tmp(st);
     int* flags = nullptr;
-    EPET_HIP(hipMallocAsync((void**)&flags, sizeof(int), st));
+    EPET_HIP(tmp.alloc(&flags, sizeof(int)));
     k_fill_i32<<<1, 64, 0, st>>>(flags, 1, 0);
     k_check_csr<<<grid_for(m, 256, 1ll << 31), 256, 0, st>>>(m, n, nnz, rowptr, col, flags);
     EPET_HIP(hipGetLastError());
-    int r = read_pinned(ctx, &hflags, flags, sizeof(int));
-    (void)hipFreeAsync(flags, st);
-    EPET_TRY(r);
+    EPET_TRY(read_pinned(ctx, &hflags, flags, sizeof(int)));
     EPET_CHECK(!(hflags & 1), EPET_ERR_ARG,
                "invalid CSR arrays (row pointers or column ids out of range)");
   }
   epet_matrix* A = nullptr;
   EPET_TRY(matrix_alloc(&A, m, n, nnz, EPET_VAL_F32));
+  MatrixGuard keep(A);
   // the tiled back projection needs ascending columns within rows
   if (hflags & 2) { A->bp_failed = true; A->cols_strict = false; }
   EPET_HIP(hipMemcpyAsync(A->rowptr, rowptr, sizeof(int64_t) * ((size_t)m + 1),
@@ -542,7 +537,7 @@ int epet_matrix_create(epet_ctx* ctx, int m, int n, long long nnz, const int64_t
     EPET_HIP(hipMemcpyAsync(A->val, val, sizeof(float) * nnz, hipMemcpyDeviceToDevice, st));
   }
   EPET_HIP(hipStreamSynchronize(st));
-  *out = A;
+  *out = keep.release();
   return EPET_OK;
 }
 
@@ -551,12 +546,13 @@ int epet_matrix_select_rows(epet_ctx* ctx, const epet_matrix* A, const int* rows
   EPET_CHECK(ctx && A && out && (nrows == 0 || rows), EPET_ERR_ARG, "null argument");
   EPET_CHECK(nrows >= 0, EPET_ERR_ARG, "negative row count");
   hipStream_t st = ctx->stream;
+  StreamTemps tmp(st);
   int* cnt = nullptr;
   int64_t* off = nullptr;
   int* bad = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&cnt, sizeof(int) * std::max(1, nrows), st));
-  EPET_HIP(hipMallocAsync((void**)&off, sizeof(int64_t) * ((size_t)nrows + 1), st));
-  EPET_HIP(hipMallocAsync((void**)&bad, sizeof(int), st));
+  EPET_HIP(tmp.alloc(&cnt, sizeof(int) * std::max(1, nrows)));
+  EPET_HIP(tmp.alloc(&off, sizeof(int64_t) * ((size_t)nrows + 1)));
+  EPET_HIP(tmp.alloc(&bad, sizeof(int)));
   k_fill_i32<<<1, 64, 0, st>>>(bad, 1, 0);
   if (nrows)
     k_sel_counts<<<grid_for(nrows, 256, 1ll << 31), 256, 0, st>>>(A->rowptr, rows, nrows, A->m,
@@ -564,14 +560,12 @@ int epet_matrix_select_rows(epet_ctx* ctx, const epet_matrix* A, const int* rows
   EPET_HIP(hipGetLastError());
   int hbad = 0;
   EPET_TRY(read_pinned(ctx, &hbad, bad, sizeof(int)));
-  if (hbad) {
-    (void)hipFreeAsync(cnt, st); (void)hipFreeAsync(off, st); (void)hipFreeAsync(bad, st);
-    EPET_FAIL(EPET_ERR_ARG, "row index out of range");
-  }
+  EPET_CHECK(!hbad, EPET_ERR_ARG, "row index out of range");
   long long nnz = 0;
   EPET_TRY(exclusive_scan_i32_to_i64(ctx, cnt, off, nrows, &nnz));
   epet_matrix* B = nullptr;
   EPET_TRY(matrix_alloc(&B, nrows, A->n, nnz, A->value_type));
+  MatrixGuard keep(B);
   if (nrows) {
     if (A->value_type == EPET_VAL_F32)
       k_sel_copy<float><<<nrows, 256, 0, st>>>(A->rowptr, A->col, (const float*)A->val,
@@ -594,20 +588,13 @@ int epet_matrix_select_rows(epet_ctx* ctx, const epet_matrix* A, const int* rows
     }
     if (B->row_ids) B->mg = A->mg;
   }
-  // last row pointer = nnz: the scan total sits one past the counts
-  int64_t* tot = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&tot, sizeof(int64_t), st));
+  // last row pointer = nnz
   {
     int64_t h = nnz;
     EPET_HIP(hipMemcpyAsync(B->rowptr + nrows, &h, sizeof(int64_t), hipMemcpyHostToDevice, st));
     EPET_HIP(hipStreamSynchronize(st));
   }
-  (void)hipFreeAsync(tot, st);
-  (void)hipFreeAsync(cnt, st);
-  (void)hipFreeAsync(off, st);
-  (void)hipFreeAsync(bad, st);
-  EPET_HIP(hipStreamSynchronize(st));
-  *out = B;
+  *out = keep.release();
   return EPET_OK;
 }
 
@@ -619,6 +606,7 @@ int epet_matrix_convert(epet_ctx* ctx, const epet_matrix* A, int vt, epet_matrix
   hipStream_t st = ctx->stream;
   epet_matrix* B = nullptr;
   EPET_TRY(matrix_alloc(&B, A->m, A->n, A->nnz, vt));
+  MatrixGuard keep(B);
   EPET_HIP(hipMemcpyAsync(B->rowptr, A->rowptr, sizeof(int64_t) * ((size_t)A->m + 1),
                           hipMemcpyDeviceToDevice, st));
   if (A->nnz) EPET_HIP(hipMemcpyAsync(B->col, A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToDevice, st));
@@ -640,7 +628,7 @@ int epet_matrix_convert(epet_ctx* ctx, const epet_matrix* A, int vt, epet_matrix
     (void)hipGetLastError();
   }
   EPET_HIP(hipStreamSynchronize(st));
-  *out = B;
+  *out = keep.release();
   return EPET_OK;
 }
 
@@ -664,10 +652,11 @@ int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A) {
   const int rows_per_block = 4096;
   const int B = std::max(1, (A->m + rows_per_block - 1) / rows_per_block);
   const long long ncnt = (long long)A->n * B;
+  StreamTemps tmp(st);
   int* cnt = nullptr;
   int64_t* off = nullptr;
-  EPET_HIP(hipMallocAsync((void**)&cnt, sizeof(int) * std::max(1ll, ncnt), st));
-  EPET_HIP(hipMallocAsync((void**)&off, sizeof(int64_t) * (ncnt + 1), st));
+  EPET_HIP(tmp.alloc(&cnt, sizeof(int) * std::max(1ll, ncnt)));
+  EPET_HIP(tmp.alloc(&off, sizeof(int64_t) * (ncnt + 1)));
   k_fill_i32<<<grid_for(ncnt, 256, 1ll << 31), 256, 0, st>>>(cnt, ncnt, 0);
   if (A->m) k_tr_count<<<B, TR_BLOCK, 0, st>>>(A->m, rows_per_block, B, A->rowptr, A->col, cnt);
   EPET_HIP(hipGetLastError());
@@ -677,6 +666,7 @@ int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A) {
   epet_matrix* T = nullptr;
   EPET_TRY(matrix_alloc(&T, A->n, A->m, A->nnz, A->value_type == EPET_VAL_F32 ? EPET_VAL_F32
                                                                               : A->value_type));
+  MatrixGuard keep(T);
   if (T->rowscale) { (void)hipFree(T->rowscale); T->rowscale = nullptr; }
   k_tr_rowptr<<<grid_for((long long)A->n + 1, 256, 1ll << 31), 256, 0, st>>>(A->n, B, A->nnz, off,
                                                                            T->rowptr);
@@ -690,12 +680,10 @@ int epet_matrix_transpose(epet_ctx* ctx, epet_matrix* A) {
                                                   (uint16_t*)T->val);
   }
   EPET_HIP(hipGetLastError());
-  (void)hipFreeAsync(cnt, st);
-  (void)hipFreeAsync(off, st);
   EPET_HIP(hipStreamSynchronize(st));
   T->bp_failed = true;
   xb_prepare(ctx, T);   // optional: without a plan the plain row kernel runs
-  A->tr = T;
+  A->tr = keep.release();
   return EPET_OK;
 }
 
