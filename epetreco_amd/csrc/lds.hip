@@ -405,7 +405,12 @@ int build_tile_major(epet_ctx* ctx, const epet_matrix* A, int TV, ColMap mp, int
       k_ysort_rows<uint16_t><<<g, 256, lds, st>>>(A->m, A->rowptr, A->col,
                                                   (const uint16_t*)A->val, mp, colp,
                                                   (uint16_t*)valp);
-    EPET_HIP(hipGetLastError());
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      (void)hipFree(colp);
+      (void)hipFree(valp);
+      EPET_FAIL(EPET_ERR_HIP, std::string("k_ysort_rows: ") + hipGetErrorString(le));
+    }
     scol = colp;
     sval = valp;
   }
