@@ -171,7 +171,9 @@ template <> struct BlkEnt<ValF16> {
     v[2] = __half2float(__ushort_as_half((uint16_t)(pk.z >> 16)));
     v[3] = __half2float(__ushort_as_half((uint16_t)(pk.w >> 16)));
     // one conversion per value, then full-rate FP32 FMAs (instead of the four
-    // mixed-precision FMAs per value the compiler would otherwise fold them into)
+    // mixed-precision FMAs per value the compiler would otherwise fold them
+    // into: 4 fewer instructions per quad, but the 32-frame forward product
+    // ran 3.6% slower that way)
     asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]));
   }
 };
