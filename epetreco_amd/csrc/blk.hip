@@ -462,14 +462,16 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
         // when the row starts, accumulated in registers and written back when
         // it ends (one slot owns a row within a tile)
         const int k0 = Lc.slot[ks];
-        int2 cr = Lc.rs[k0 + 1];
+        const int2 c0r = Lc.rs[k0 + 1];
+        unsigned ra = (unsigned)c0r.x + lane_off;   // this lane's part of the row's sums
+        int rend = c0r.y;
         int2 nx = Lc.rs[k0 + 2];
         const int2* kp = Lc.rs + k0 + 3;
         float a4[P];
         auto load_row = [&]() {
 #pragma unroll
           for (int i = 0; i < P / 4; i++) {
-            const float4 o = *reinterpret_cast<const float4*>(xb + cr.x + lane_off + 16 * i);
+            const float4 o = *reinterpret_cast<const float4*>(xb + ra + 16 * i);
             a4[4 * i + 0] = o.x;
             a4[4 * i + 1] = o.y;
             a4[4 * i + 2] = o.z;
@@ -503,14 +505,15 @@ __global__ __launch_bounds__(BLK_THREADS) void k_blk(BlkArgs a) {
                 a4[4 * i + 3] += v[t4] * xv[t4][i].w;
               }
             e += 4;
-            if (e == cr.y) {
+            if (e == rend) {
               // the row is complete in this tile
-              float4* dst = reinterpret_cast<float4*>(const_cast<char*>(xb) + cr.x + lane_off);
+              float4* dst = reinterpret_cast<float4*>(const_cast<char*>(xb) + ra);
 #pragma unroll
               for (int i = 0; i < P / 4; i++)
                 dst[i] = make_float4(a4[4 * i + 0], a4[4 * i + 1], a4[4 * i + 2], a4[4 * i + 3]);
               if (e < e_end) {
-                cr = nx;
+                ra = (unsigned)nx.x + lane_off;
+                rend = nx.y;
                 nx = *kp++;
                 load_row();
               }
