@@ -89,3 +89,15 @@ def test_phantoms_are_seeded_and_in_range():
     imgs, totals = PH.dynamic_series(G.grid_16_3mm(), "small", 32, 0)
     assert imgs.shape == (32, 4096) and np.all(totals >= 1e5) and np.all(totals <= 1e7)
     assert np.all(np.diff(imgs.sum(1)) <= 0)          # activities decay with t
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and a JSON line
+    that says so, before touching a device."""
+    import json
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
